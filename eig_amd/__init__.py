@@ -1,0 +1,181 @@
+"""MI355X smartpca engine: Python binding of libeig_amd.so (the C-ABI in
+include/eig_abi.h).  The compute path is the HIP library; this module only moves
+arrays across the boundary.  It raises if the library is missing -- there is no
+CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libeig_amd.so")
+CLI_PATH = os.path.join(PKG, "bin", "smartpca")
+
+FORM_FP64 = 0
+FORM_BF16X3 = 1
+SNP_KEPT, SNP_DROPPED, SNP_ALLMISS, SNP_DEAD = 0, 1, 2, 3
+
+_lib = None
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+# name -> (restype, argtypes); mirrors include/eig_abi.h
+SIGNATURES = {
+    "eig_open": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "eig_close": (None, [_vp]),
+    "eig_last_error": (C.c_char_p, [_vp]),
+    "eig_last_ms": (C.c_double, [_vp, C.c_char_p]),
+    "eig_load_packed": (C.c_int, [_vp, _u8p, C.c_int64, C.c_int64, C.c_int32]),
+    "eig_synth_packed": (C.c_int, [_vp, C.c_int64, C.c_int32, C.c_int32, _u32p, _i32p, _u32p,
+                                   C.c_uint32, C.c_uint64]),
+    "eig_fetch_packed": (C.c_int, [_vp, C.c_int64, C.c_int64, _u8p]),
+    "eig_select": (C.c_int, [_vp, _i32p, C.c_int32]),
+    "eig_snpstats": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _i32p,
+                               _i32p, _i32p, _f64p, _f64p, _i32p]),
+    "eig_xtx": (C.c_int, [_vp, C.c_int32, C.c_int32, C.POINTER(C.c_double), _vp]),
+    "eig_eigh_xtx": (C.c_int, [_vp, C.c_int32, _f64p, _vp]),
+    "eig_eigh": (C.c_int, [_vp, _f64p, C.c_int32, C.c_int32, _f64p, _vp]),
+    "eig_loadings": (C.c_int, [_vp, _f64p, C.c_int32, _vp, _vp, _vp]),
+    "eig_lsqproject": (C.c_int, [_vp, _i32p, _f64p, _f64p, _i32p]),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is not built: run python -m eig_amd.build (no CPU fallback exists)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class EigError(RuntimeError):
+    pass
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Engine:
+    """One device context (eig_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self._L = lib()
+        h = _vp()
+        rc = self._L.eig_open(device, C.byref(h))
+        if rc != 0:
+            raise EigError(f"eig_open({device}) failed with {rc}")
+        self.h = h
+        self.nsnp = 0
+        self.nind = 0
+        self.nrows = 0
+
+    def close(self):
+        if self.h:
+            self._L.eig_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self._L.eig_last_error(self.h)
+            raise EigError(f"{what}: rc={rc}: {msg.decode() if msg else ''}")
+
+    def last_ms(self, what: str) -> float:
+        return self._L.eig_last_ms(self.h, what.encode())
+
+    def load_packed(self, rows: np.ndarray, nind: int):
+        rows = np.ascontiguousarray(rows, dtype=np.uint8)
+        self._chk(self._L.eig_load_packed(self.h, rows.reshape(-1), rows.shape[1], rows.shape[0],
+                                          nind), "eig_load_packed")
+        self.nsnp, self.nind = rows.shape[0], nind
+
+    def synth(self, nsnp, nind, thr, anc, cum, miss_thr, seed):
+        thr = np.ascontiguousarray(thr, dtype=np.uint32)
+        npop = thr.shape[0]
+        self._chk(self._L.eig_synth_packed(self.h, nsnp, nind, npop, thr.reshape(-1),
+                                           np.ascontiguousarray(anc, dtype=np.int32).reshape(-1),
+                                           np.ascontiguousarray(cum, dtype=np.uint32).reshape(-1),
+                                           int(miss_thr), int(seed)), "eig_synth_packed")
+        self.nsnp, self.nind = nsnp, nind
+
+    def fetch_packed(self, m0=0, count=None):
+        if count is None:
+            count = self.nsnp - m0
+        rlen = max(48, (2 * self.nind + 7) // 8)
+        out = np.zeros((count, rlen), dtype=np.uint8)
+        self._chk(self._L.eig_fetch_packed(self.h, m0, count, out.reshape(-1)), "eig_fetch_packed")
+        return out
+
+    def select(self, xindex):
+        xi = np.ascontiguousarray(xindex, dtype=np.int32)
+        self._chk(self._L.eig_select(self.h, xi, len(xi)), "eig_select")
+        self.nrows = len(xi)
+
+    def snpstats(self, live=None, altnorm=1, fancynorm=1, minallelecnt=1, maxmissing=9999999):
+        M = self.nsnp
+        out = dict(c0=np.zeros(M, np.int32), c1=np.zeros(M, np.int32),
+                   nmiss=np.zeros(M, np.int32), mean=np.zeros(M), fancy=np.zeros(M),
+                   status=np.zeros(M, np.int32))
+        lv = None if live is None else np.ascontiguousarray(live, dtype=np.uint8)
+        self._chk(self._L.eig_snpstats(self.h, _ptr(lv), altnorm, fancynorm, minallelecnt,
+                                       maxmissing, out["c0"], out["c1"], out["nmiss"],
+                                       out["mean"], out["fancy"], out["status"]), "eig_snpstats")
+        return out
+
+    def xtx(self, form=FORM_FP64, block=0, want_matrix=True):
+        y = C.c_double(0)
+        mat = np.zeros((self.nrows, self.nrows)) if want_matrix else None
+        self._chk(self._L.eig_xtx(self.h, form, block, C.byref(y), _ptr(mat)), "eig_xtx")
+        return y.value, mat
+
+    def eigh_xtx(self, nvec):
+        ev = np.zeros(self.nrows)
+        vec = np.zeros((nvec, self.nrows)) if nvec > 0 else None
+        self._chk(self._L.eig_eigh_xtx(self.h, nvec, ev, _ptr(vec)), "eig_eigh_xtx")
+        return ev, vec
+
+    def eigh(self, a, nvec):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        n = a.shape[0]
+        ev = np.zeros(n)
+        vec = np.zeros((nvec, n)) if nvec > 0 else None
+        self._chk(self._L.eig_eigh(self.h, a.reshape(-1), n, nvec, ev, _ptr(vec)), "eig_eigh")
+        return ev, vec
+
+    def loadings(self, evecs, ignore=None, want_ffvecs=True):
+        ev = np.ascontiguousarray(evecs, dtype=np.float64)
+        K = ev.shape[0]
+        ign = None if ignore is None else np.ascontiguousarray(ignore, dtype=np.uint8)
+        ff = np.zeros((K, self.nsnp)) if want_ffvecs else None
+        fx = np.zeros(K)
+        self._chk(self._L.eig_loadings(self.h, ev.reshape(-1), K, _ptr(ign), _ptr(ff), _ptr(fx)),
+                  "eig_loadings")
+        return ff, fx
+
+    def lsqproject(self, use, K):
+        use = np.ascontiguousarray(use, dtype=np.int32)
+        n = len(use)
+        a = np.zeros((K, n)); b = np.zeros((K, n)); st = np.zeros(n, np.int32)
+        self._chk(self._L.eig_lsqproject(self.h, use, a.reshape(-1), b.reshape(-1), st),
+                  "eig_lsqproject")
+        return a, b, st

@@ -1,0 +1,104 @@
+// Internal definitions shared by the HIP translation units of libeig_amd.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+
+#include "../../include/eig_abi.h"
+
+// Sample tile of the selected-genotype layout: 128 samples = 8 x 32-bit words of
+// 2-bit codes (sample 16w+i of the tile at bits 2i..2i+1 of word w).
+constexpr int kTileS = 128;
+constexpr int kTileWords = kTileS / 16;
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+struct eig_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  std::map<std::string, double> ms;
+  int num_cu = 0;
+
+  // genotypes as loaded (GENO row layout)
+  DevBuf geno;
+  int64_t rlen = 0, nsnp = 0;
+  int32_t nind = 0;
+
+  // selection of axis samples and its tile layout sel[(st*nsnp + m)*8 + w]
+  DevBuf xindex, sel;
+  int32_t nrows = 0, nst = 0;
+  bool have_sel = false;
+
+  // per-SNP statistics (device)
+  DevBuf c0, c1, nmiss, status, mean, fancy;
+  DevBuf lev;    // double [nsnp][4] : levels of g = 0,1,2 and 0 for missing
+  DevBuf levbf;  // uint32 [nsnp][4] : bf16 hi (low half) | bf16 lo (high half)
+  DevBuf live;   // uint8 [nsnp]
+  bool have_stats = false;
+
+  // XTX (normalised, row-major nrows x nrows) and partial sums
+  DevBuf xtx, part;
+  bool have_xtx = false;
+
+  // eigensolver
+  DevBuf ework;
+  DevBuf evecs_dev;  // nvec x nrows of the last eig_eigh_xtx
+  int32_t last_nvec = 0;
+
+  // loadings / projection
+  DevBuf ffvecs, fxscal, projw, proj_out, use_dev, ignore_dev;
+  int32_t last_K = 0;
+  bool have_load = false;
+
+  // generic scratch
+  DevBuf scratch, scratch2, hostpin;
+};
+
+int eig_set_err(eig_ctx *ctx, int code, const char *fmt, ...);
+int eig_ensure(eig_ctx *ctx, DevBuf &b, size_t bytes);
+void eig_time_begin(eig_ctx *ctx);
+void eig_time_end(eig_ctx *ctx, const char *what);
+
+#define EIG_HIP(ctx, call)                                                        \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return eig_set_err((ctx), EIG_E_HIP, "%s:%d %s: %s", __FILE__, __LINE__,    \
+                         #call, hipGetErrorString(e_));                           \
+  } while (0)
+
+#define EIG_TRY(expr)           \
+  do {                          \
+    int r_ = (expr);            \
+    if (r_ != EIG_OK) return r_; \
+  } while (0)
+
+// ---- device helpers ---------------------------------------------------------
+__device__ __host__ inline uint16_t eig_bf16_rne(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf/nan
+  uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(r >> 16);
+}
+
+__device__ __host__ inline float eig_bf16_to_f32(uint16_t h) {
+  uint32_t u = (uint32_t)h << 16;
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+
+// internal launchers implemented in the kernel translation units
+int eig_run_syrk(eig_ctx *ctx, int form, int bf16_block, double *trace_scale);
+int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, double *evals,
+                 double *evecs);

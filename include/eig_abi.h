@@ -1,0 +1,126 @@
+/*
+ * eig_abi.h -- C-ABI of the MI355X smartpca engine (libeig_amd.so).
+ *
+ * Plain C: pointers, sizes and error codes only.  Every entry point returns
+ * EIG_OK (0) or a negative EIG_E_* code; eig_last_error() gives the message.
+ * A context is used from one host thread at a time; all device work of a
+ * context runs on its own HIP stream (default priority).
+ *
+ * The reference (EIGENSOFT 8.0 smartpca) has no plugin or FFI API: its only
+ * public contract is the `smartpca -p parfile` process.  The entry points below
+ * replace the internal seams of that process; each cites the reference code it
+ * stands in for.  Genotype codes follow the reference: 0/1/2 = count of the
+ * first allele, 3 = missing (admutils.c:575-592).
+ */
+#ifndef EIG_ABI_H
+#define EIG_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EIG_OK 0
+#define EIG_E_ARG (-1)     /* bad argument / size                                  */
+#define EIG_E_HIP (-2)     /* HIP runtime error                                    */
+#define EIG_E_NOMEM (-3)   /* device or host allocation failed                     */
+#define EIG_E_NUMERIC (-4) /* NaN or non-positive XTX trace (smartpca.c:1230-1235) */
+#define EIG_E_STATE (-5)   /* call order (e.g. eig_xtx before eig_select)          */
+#define EIG_E_CONVERGE (-6)/* eigensolver did not converge                          */
+
+/* XTX accumulation forms */
+#define EIG_FORM_FP64 0   /* FP64 end to end (MFMA f64)                             */
+#define EIG_FORM_BF16X3 1 /* split-BF16 hi+lo, 3 MFMA products, FP32 per SNP block,
+                             FP64 across blocks                                     */
+
+/* per-SNP status written by eig_snpstats */
+#define EIG_SNP_KEPT 0
+#define EIG_SNP_DROPPED 1  /* min(c0,c1) < minallelecnt, == 0, or nmiss > maxmissing */
+#define EIG_SNP_ALLMISS 2  /* no observed genotype among the axis samples           */
+#define EIG_SNP_DEAD 3     /* not live in this pass (caller's mask)                 */
+
+typedef struct eig_ctx eig_ctx;
+
+/* Open a context on HIP device `device`. */
+int eig_open(int device, eig_ctx **out);
+void eig_close(eig_ctx *ctx);
+const char *eig_last_error(const eig_ctx *ctx);
+/* Wall time in ms of the last device phase named by `what` ("xtx", "eigh",
+ * "tridiag", "syrk", ...), measured with HIP events on the context stream; -1 if
+ * not measured. */
+double eig_last_ms(const eig_ctx *ctx, const char *what);
+
+/* Copy packed genotype rows (GENO layout: nsnp records of rlen bytes, sample k in
+ * byte k/4 bits 7-6,5-4,3-2,1-0) to the device.  Replaces inpack's in-memory
+ * `packgenos` buffer (mcio.c:2769-2879).  rlen >= ceil(nind/4). */
+int eig_load_packed(eig_ctx *ctx, const uint8_t *rows, int64_t rlen, int64_t nsnp,
+                    int32_t nind);
+
+/* Generate seeded synthetic genotypes directly in device memory (Balding-Nichols
+ * frequencies prepared by the caller).  thr[pop*nsnp + m] is the allele-frequency
+ * threshold p*2^32 of population pop at SNP m; sample i draws each allele copy
+ * from population anc[3i+c] with cumulative ancestry weights cum[2i], cum[2i+1]
+ * (uint32 fractions of 2^32); missing when a third draw falls below miss_thr.
+ * The same integer recipe is restated in eig_amd/synth.py for CPU checks. */
+int eig_synth_packed(eig_ctx *ctx, int64_t nsnp, int32_t nind, int32_t npop,
+                     const uint32_t *thr, const int32_t *anc, const uint32_t *cum,
+                     uint32_t miss_thr, uint64_t seed);
+/* Copy device genotype rows [m0, m0+count) back to the host (rlen bytes each). */
+int eig_fetch_packed(eig_ctx *ctx, int64_t m0, int64_t count, uint8_t *rows);
+
+/* Choose the axis samples of a pass: xindex[0..nrows) indexes the nind samples of
+ * the loaded rows, in increasing order (loadindx, qpsubs.c:202-219).  Builds the
+ * device tile layout the column statistics and XTX kernels read. */
+int eig_select(eig_ctx *ctx, const int32_t *xindex, int32_t nrows);
+
+/* Column statistics of every SNP over the selected samples: getcolxz_binary1 +
+ * fvadjust_binary (smartpca.c:3219-3301, 2282-2313) and the keep rule
+ * (smartpca.c:1131-1144).  live[m] == 0 marks a SNP already removed (status
+ * EIG_SNP_DEAD, contributes nothing); live may be NULL.  Output arrays have nsnp
+ * entries and may each be NULL.  Leaves the per-SNP level tables on the device
+ * for eig_xtx / eig_loadings / eig_lsqproject. */
+int eig_snpstats(eig_ctx *ctx, const uint8_t *live, int32_t altnorm, int32_t fancynorm,
+                 int32_t minallelecnt, int32_t maxmissing, int32_t *c0, int32_t *c1,
+                 int32_t *nmiss, double *mean, double *fancy, int32_t *status);
+
+/* XTX over the kept SNPs and selected samples, then symmetric unpack and division
+ * by trace/(nrows-1): domult_increment_lookup + symit2 + trace scaling
+ * (smartpca.c:3426-3495, 480-508, 1222-1236).  The normalised matrix stays on the
+ * device for eig_eigh_xtx; xtx_out (nrows*nrows, row-major) may be NULL.
+ * *trace_scale receives trace/(nrows-1) before scaling.  bf16_block is the FP32
+ * accumulation length in SNPs for EIG_FORM_BF16X3 (multiple of 64; 0 = 4096). */
+int eig_xtx(eig_ctx *ctx, int32_t form, int32_t bf16_block, double *trace_scale,
+            double *xtx_out);
+
+/* Eigen-decomposition of the device-resident normalised XTX: eigvecs
+ * (eigsubs.c:39-55) without its LAPACK call.  evals (nrows, descending) get every
+ * eigenvalue; evecs (nvec x nrows, row j = eigenvector j, unit norm) the top
+ * nvec eigenvectors, each with its largest-|.| component made positive. */
+int eig_eigh_xtx(eig_ctx *ctx, int32_t nvec, double *evals, double *evecs);
+
+/* Same, for a caller's symmetric row-major n x n matrix a (host memory). */
+int eig_eigh(eig_ctx *ctx, const double *a, int32_t n, int32_t nvec, double *evals,
+             double *evecs);
+
+/* SNP loadings and projection scales from the top-K eigenvectors of the last
+ * eig_eigh_xtx: ffvecs[j][m] = sum_i 10 v_j[i] z_m[i] over SNPs with ignore[m]==0
+ * (setfvecs qpsubs.c:489-508, smartpca.c:1438-1494, getcolxf 3564-3595), and
+ * fxscal_j = 1/sqrt(sum_i <x_i, ffvecs_j>^2) over the selected samples
+ * (smartpca.c:1504-1525).  ignore has nsnp entries (NULL = use status != KEPT).
+ * evecs: K x nrows host array.  ffvecs (K x nsnp) and fxscal (K) may be NULL. */
+int eig_loadings(eig_ctx *ctx, const double *evecs, int32_t K, const uint8_t *ignore,
+                 double *ffvecs, double *fxscal);
+
+/* lsqproject of every individual with use[q] != 0 (q over the nind loaded
+ * samples): lsqproj + regressit + solvit (smartpca.c:4606-4757, regsubs.c:8-74,
+ * linsubs.c:296-393).  acoeffs / bcoeffs: K x nind (row j at j*nind).  status[q]:
+ * 0 solved, 1 ignored (<= K observed SNPs, coefficients untouched), 2 Cholesky
+ * failed (previous individual's answer kept, as the reference does), 3 not used. */
+int eig_lsqproject(eig_ctx *ctx, const int32_t *use, double *acoeffs, double *bcoeffs,
+                   int32_t *status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EIG_ABI_H */
