@@ -35,10 +35,13 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+BF16_BLOCK = 256  # SNPs per FP32 accumulation block of the split-BF16 forms
+
+
 def one_pass(eng, xindex, form, nvec):
     eng.select(xindex)
     eng.snpstats()
-    eng.xtx(form, 4096, want_matrix=False)
+    eng.xtx(form, BF16_BLOCK, want_matrix=False)
     ev, vec = eng.eigh_xtx(nvec)
     return ev, vec
 
@@ -93,7 +96,9 @@ def main():
     P.to_device(eng)
     xindex = np.arange(N, dtype=np.int32)
     res = {}
-    for form, name in ((eig_amd.FORM_BF16X3, "bf16x3"), (eig_amd.FORM_FP64, "fp64")):
+    forms = ((eig_amd.FORM_BF16X4, "bf16x4"), (eig_amd.FORM_BF16X3, "bf16x3"),
+             (eig_amd.FORM_FP64, "fp64"))
+    for form, name in forms:
         for _ in range(args.warmup):
             one_pass(eng, xindex, form, nvec)
         times, syrk_ms, eig_ms = [], [], []
@@ -102,38 +107,50 @@ def main():
             ev, vec = one_pass(eng, xindex, form, nvec)
             times.append(time.perf_counter() - t0)
             syrk_ms.append(eng.last_ms("syrk"))
-            eig_ms.append(eng.last_ms("tridiag") + eng.last_ms("bisect") + eng.last_ms("eigvec"))
+            eig_ms.append(sum(eng.last_ms(k) for k in ("tridiag", "bisect", "invit", "eigvec")))
             log(f"{name} step {s}: {times[-1]*1e3:.1f} ms (syrk {syrk_ms[-1]:.2f} ms, "
                 f"eig {eig_ms[-1]:.2f} ms)")
         res[name] = dict(t=float(np.mean(times)), syrk_ms=float(np.mean(syrk_ms)),
                          eig_ms=float(np.mean(eig_ms)), ev=ev, vec=vec,
                          stats_ms=eng.last_ms("snpstats"), select_ms=eng.last_ms("select"),
-                         parts={k: eng.last_ms(k) for k in ("tridiag", "bisect", "eigvec",
+                         parts={k: eng.last_ms(k) for k in ("tridiag", "bisect", "invit", "eigvec",
                                                           "xtx_finalize")})
-    b, f = res["bf16x3"], res["fp64"]
+    b, b3, f = res["bf16x4"], res["bf16x3"], res["fp64"]
     top = slice(0, nvec)
-    relerr = float(np.max(np.abs(b["ev"][top] - f["ev"][top]) / np.abs(f["ev"][top])))
-    cosang = np.abs(np.sum(b["vec"] * f["vec"], axis=1))
-    flops = 3.0 * N * (N + 1) * M
+
+    def errs(x):
+        rel = float(np.max(np.abs(x["ev"][top] - f["ev"][top]) / np.abs(f["ev"][top])))
+        cos = float(np.abs(np.sum(x["vec"] * f["vec"], axis=1)).min())
+        allabs = float(np.max(np.abs(x["ev"] - f["ev"])))
+        return rel, cos, allabs
+    relerr, cosmin, allabs = errs(b)
+    relerr3, cosmin3, allabs3 = errs(b3)
+    flops = 4.0 * N * (N + 1) * M
     ach = flops / (b["syrk_ms"] * 1e-3) / 1e12
     out = {
         "metric": METRIC, "value": round(b["t"], 5), "unit": "s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(b["t"] * 1e3, 3),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16 (hi+lo x3 MFMA, fp32 per 4096-SNP block, fp64 across; fp64 eigensolve)",
+        "dtype": "bf16 (hi+lo, 4 MFMA products, fp32 per 256-SNP block, fp64 across; fp64 eigensolve)",
         "data": "synthetic",
         "config": {"workload": "XTX+eigensolve, Balding-Nichols 10 pops x 500, 2% missing",
-                   "N": N, "M": M, "numoutevec": nvec, "bf16_block": 4096},
+                   "N": N, "M": M, "numoutevec": nvec, "bf16_block": BF16_BLOCK,
+                   "form": "split-BF16 hi+lo, 4 products"},
         "fp64_s": round(f["t"], 5),
         "speedup_vs_fp64": round(f["t"] / b["t"], 3),
         "top10_eval_relerr_vs_fp64": relerr,
-        "top10_evec_min_abs_cos_vs_fp64": float(cosang.min()),
-        "stages_ms": {"bf16": {"select": b["select_ms"], "snpstats": b["stats_ms"],
-                               "syrk": b["syrk_ms"], **b["parts"]},
+        "top10_evec_min_abs_cos_vs_fp64": cosmin,
+        "all_eval_max_abs_err_vs_fp64": allabs,
+        "bf16x3_s": round(b3["t"], 5),
+        "bf16x3_top10_eval_relerr_vs_fp64": relerr3,
+        "bf16x3_all_eval_max_abs_err_vs_fp64": allabs3,
+        "stages_ms": {"bf16x4": {"select": b["select_ms"], "snpstats": b["stats_ms"],
+                                 "syrk": b["syrk_ms"], **b["parts"]},
+                      "bf16x3": {"syrk": b3["syrk_ms"], **b3["parts"]},
                       "fp64": {"syrk": f["syrk_ms"], **f["parts"]}},
         "roofline": {"bound": "mfma", "achieved": round(ach, 2), "peak": PEAK_BF16,
                      "unit": "TFLOP/s", "frac": round(ach / PEAK_BF16, 4), "traffic": None,
-                     "kernel": "k_syrk_bf16x3", "algorithmic_flops": flops},
+                     "kernel": "k_syrk_bf16x3<true> (4 products)", "algorithmic_flops": flops},
         "fp64_syrk_roofline": {"achieved": round(N * (N + 1) * M / (f["syrk_ms"] * 1e-3) / 1e12, 2),
                                "peak": PEAK_FP64, "unit": "TFLOP/s"},
     }

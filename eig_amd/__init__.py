@@ -16,6 +16,7 @@ CLI_PATH = os.path.join(PKG, "bin", "smartpca")
 
 FORM_FP64 = 0
 FORM_BF16X3 = 1
+FORM_BF16X4 = 2
 SNP_KEPT, SNP_DROPPED, SNP_ALLMISS, SNP_DEAD = 0, 1, 2, 3
 
 _lib = None

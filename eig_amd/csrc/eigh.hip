@@ -1,218 +1,289 @@
 // FP64 symmetric eigensolver on the device.
 //
-//  1. Householder tridiagonalisation, blocked by panels of kNB columns
-//     (the dsytrd/dlatrd scheme).  The matrix is row-major and symmetric, so
-//     "column c below the diagonal" is read as row c right of the diagonal and
-//     every vector access is contiguous.  Per column three kernels run:
-//       k_col  : row c minus the panel's rank-2j correction, Householder scalars
-//       k_symv : y = A_trail v, plus the panel dot products V v and W v
-//       k_wvec : w = tau (y - V (W v) - W (V v)), then alpha = -tau/2 w.v
-//     Cross-block reductions use the last-arriving block (atomic ticket), so no
-//     kernel ever waits on another block.  After each panel k_syr2k applies
-//     A -= V W^T + W V^T to the trailing matrix.  The Householder vector of row
-//     c is left in A[c][c+1:].
-//  2. All eigenvalues of the tridiagonal T by bisection on Sturm counts, one
-//     thread per eigenvalue.
+//  1. Householder tridiagonalisation, blocked by panels of kNB columns (the
+//     dsytrd/dlatrd scheme).  The working matrix is row-major and symmetric with
+//     a padded leading dimension, so "column c below the diagonal" is row c right
+//     of the diagonal and every access is a contiguous 16-byte pair.  Two kernels
+//     run per column c (panel position j):
+//       k_symv : y = A_trail v with v rebuilt on the fly from the unscaled
+//                reflector column x and its scale; the panel dots p = V^T v,
+//                q = W^T v and y.v are reduced across blocks.
+//       k_wcol : w = tau (y - V q - W p) + alpha v, where alpha = -tau/2 w.v is
+//                known up front from w.v = tau (y.v - 2 p.q); in the same pass the
+//                next row x = A[c+1] - (V W^T + W V^T)[c+1] and its norm; the last
+//                block forms the next reflector (dlarfg).
+//     Reductions across blocks are two-level and in fixed order (the last block
+//     of each group of 16, then the last group), so results do not depend on
+//     scheduling and no block ever waits for another.  After each panel k_syr2k
+//     applies A -= V W^T + W V^T to the trailing matrix.  The reflector of row c
+//     is left in A[c][c+1:], with tau[c].  The launch sequence for one n is
+//     captured once into a hipGraph and replayed.
+//  2. All eigenvalues of the tridiagonal T by 33-way multisection on Sturm counts
+//     (16 lanes per eigenvalue, two interleaved counts per lane).
 //  3. The top nvec eigenvectors of T by inverse iteration (LU with partial
-//     pivoting), with Gram-Schmidt inside clusters of close eigenvalues.
-//  4. Back-transform X = Q Y panel by panel with the compact-WY form
-//     Q_p = I - V_p T_p V_p^T, last panel first.
+//     pivoting), Gram-Schmidt inside clusters of close eigenvalues.
+//  4. Back-transform X = Q Y in compact-WY form Q_p = I - V_p T_p V_p^T, last
+//     panel first; all T_p are formed up front in one launch, then one fused
+//     kernel per panel applies the previous panel and reduces V_p^T X.
 #include <cmath>
 #include <cstring>
 #include <vector>
 
 #include "eig_internal.h"
 
-constexpr int kNB = 32;       // panel width
-constexpr int kMaxVec = 64;   // max eigenvectors computed
+constexpr int kNB = 32;      // panel width
+constexpr int kMaxVec = 64;  // max eigenvectors computed
+constexpr int kGroup = 16;   // blocks per first-level reduction group
 
-// scalar slots of the solver state (doubles)
 enum {
-  S_TAU = 0,      // tau of the current column
-  S_SCALE,        // 1/(alpha-beta) of the current column
-  S_ALPHAW,       // alpha of the current w (w_final = w_pre + alpha v)
-  S_GL, S_GU, S_PIVMIN, S_TNORM,
-  S_PD = 8,       // V_q . v  (kNB)
-  S_QD = 8 + kNB, // W_q . v  (kNB)
+  S_TAU = 0,  // tau of the current column
+  S_SCALE,    // 1/(alpha-beta) of the current column
+  S_YV,       // y.v
+  S_GL, S_GU, S_PIVMIN, S_TNORM, S_NORM,
+  S_PD = 8,        // p_q = V_q . v  (kNB)
+  S_QD = 8 + kNB,  // q_q = W_q . v  (kNB)
   S_COUNT = 8 + 2 * kNB
 };
 
 struct EigWs {
-  double *a;       // n x n working matrix (destroyed)
+  double *a;  // n x lda working matrix (destroyed); columns n..lda are zero
+  int32_t lda;
   double *d, *e, *e2, *tau;
-  double *V, *W;   // kNB x n panel vectors
-  double *xtmp, *wtmp, *y;
-  double *sc;      // scalars
-  double *part;    // block partials
-  unsigned *ticket;
-  double *eval;    // ascending
-  double *X;       // nvec x n eigenvectors
-  double *lu;      // nvec x 5n scratch for inverse iteration
-  double *Z;       // kNB x kMaxVec
+  double *V, *W;  // kNB x n panel vectors
+  double *xtmp;   // current reflector column, unscaled (length lda, zero padded)
+  double *y;
+  double *sc;
+  double *part1, *part2;  // two-level reduction partials
+  unsigned *ticket;       // [0] final, [1 + g] group counters
+  double *eval;           // ascending
+  double *X;              // nvec x n eigenvectors
+  double *lu;             // nvec x 5n scratch for inverse iteration
+  double *Z;              // kNB x kMaxVec
+  double *Tall;           // panels x kNB x kNB
 };
 
-__device__ inline bool last_block(unsigned *ticket) {
-  __shared__ bool amlast;
+// Deterministic two-level reduction of nv values per block (vals in shared
+// memory).  Returns true in exactly one block, whose `out` (shared, nv; may be
+// the same array as vals) then holds the sums over all blocks in block order.
+// Only wave 0 touches the partial buffers, so the release/acquire of the ticket
+// atomics (issued by its lane 0) covers every cross-block byte; the callers
+// exchange nothing else between blocks inside one kernel.
+__device__ bool hreduce(EigWs &ws, const double *vals, int nv, double *out) {
+  __shared__ bool g_last, f_last;
+  const unsigned b = blockIdx.x, nb = gridDim.x;
+  const unsigned g = b / kGroup, ng = (nb + kGroup - 1) / kGroup;
+  const unsigned gsz = min((unsigned)kGroup, nb - g * kGroup);
+  const bool w0 = threadIdx.x < 64;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    amlast = (prev == gridDim.x - 1);
+  if (w0) {
+    for (int t = threadIdx.x; t < nv; t += 64) ws.part1[(int64_t)b * nv + t] = vals[t];
+    if (threadIdx.x == 0) {
+      unsigned prev = __hip_atomic_fetch_add(&ws.ticket[1 + g], 1u, __ATOMIC_ACQ_REL,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+      g_last = (prev == gsz - 1);
+    }
   }
   __syncthreads();
-  if (amlast) __threadfence();
-  return amlast;
+  if (!g_last) return false;
+  if (w0) {
+    __threadfence();
+    for (int t = threadIdx.x; t < nv; t += 64) {
+      double s = 0.0;
+      for (unsigned k = 0; k < gsz; ++k) s += ws.part1[(int64_t)(g * kGroup + k) * nv + t];
+      ws.part2[(int64_t)g * nv + t] = s;
+    }
+    if (threadIdx.x == 0) {
+      ws.ticket[1 + g] = 0;
+      unsigned prev = __hip_atomic_fetch_add(&ws.ticket[0], 1u, __ATOMIC_ACQ_REL,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+      f_last = (prev == ng - 1);
+    }
+  }
+  __syncthreads();
+  if (!f_last) return false;
+  if (w0) {
+    __threadfence();
+    for (int t = threadIdx.x; t < nv; t += 64) {
+      double s = 0.0;
+      for (unsigned k = 0; k < ng; ++k) s += ws.part2[(int64_t)k * nv + t];
+      out[t] = s;
+    }
+    if (threadIdx.x == 0) ws.ticket[0] = 0;
+  }
+  __syncthreads();
+  return true;
 }
 
+// sum over a 256-thread block; result valid in every thread
 __device__ inline double block_sum(double v, double *red) {
-  // 256 threads
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  __syncthreads();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
-  double s = 0.0;
-  if (threadIdx.x == 0) s = (red[0] + red[1]) + (red[2] + red[3]);
-  __syncthreads();
-  return s;
+  return (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-// ---------------------------------------------------------------- k_col
-// Row c after the panel correction -> xtmp[c..n); finalises W[j-1] = wtmp +
-// alpha v on this block's range; last block forms the Householder reflector.
-__global__ __launch_bounds__(256) void k_col(EigWs ws, int32_t n, int32_t c, int32_t j) {
-  __shared__ double red[4];
-  const int32_t s = c + blockIdx.x * 256 + threadIdx.x;
-  double x = 0.0;
-  double nrm = 0.0;
-  if (s < n) {
-    double aw = 0.0;
-    if (j > 0) {
-      // W[j-1] on the fly (race-free: wtmp and V are not written here)
-      const double al = ws.sc[S_ALPHAW];
-      const double *vj = ws.V + (int64_t)(j - 1) * n, *wj = ws.wtmp;
-      double wc = wj[c] + al * vj[c];
-      double ws_ = wj[s] + al * vj[s];
-      ws.W[(int64_t)(j - 1) * n + s] = ws_;
-      x = ws.a[(int64_t)c * n + s];
-      for (int q = 0; q < j - 1; ++q) {
-        const double *vq = ws.V + (int64_t)q * n, *wq = ws.W + (int64_t)q * n;
-        x -= vq[c] * wq[s] + wq[c] * vq[s];
-      }
-      x -= vj[c] * ws_ + wc * vj[s];
-      (void)aw;
-    } else {
-      x = ws.a[(int64_t)c * n + s];
-    }
-    ws.xtmp[s] = x;
-    if (s == c) ws.d[c] = x;
-    if (s >= c + 2) nrm = x * x;
+// dlarfg for column c from alpha = x[c+1] and xn2 = sum_{r>c+1} x^2.
+__device__ inline void reflector_scalars(EigWs &ws, int32_t c, double alpha, double xn2) {
+  double tau = 0.0, scale = 0.0, beta = alpha;
+  if (xn2 > 0.0) {
+    beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+    tau = (beta - alpha) / beta;
+    scale = 1.0 / (alpha - beta);
   }
-  double bs = block_sum(nrm, red);
-  if (threadIdx.x == 0) ws.part[blockIdx.x] = bs;
-  if (!last_block(ws.ticket)) return;
+  ws.e[c] = beta;
+  ws.e2[c] = beta * beta;
+  ws.tau[c] = tau;
+  ws.sc[S_TAU] = tau;
+  ws.sc[S_SCALE] = scale;
+}
+
+// column 0: x = row 0, d[0], reflector (one block)
+__global__ __launch_bounds__(256) void k_col0(EigWs ws, int32_t n) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (int32_t r = threadIdx.x; r < n; r += 256) {
+    const double x = ws.a[r];
+    ws.xtmp[r] = x;
+    if (r >= 2) s += x * x;
+  }
+  s = block_sum(s, red);
   if (threadIdx.x == 0) {
-    double xn2 = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b) xn2 += ws.part[b];
-    ws.ticket[0] = 0;
-    if (c + 1 < n) {
-      double alpha = ws.xtmp[c + 1];
-      double tau = 0.0, scale = 0.0, beta = alpha;
-      if (xn2 > 0.0) {
-        double xn = sqrt(xn2);
-        beta = -copysign(sqrt(alpha * alpha + xn * xn), alpha);
-        tau = (beta - alpha) / beta;
-        scale = 1.0 / (alpha - beta);
-      }
-      ws.e[c] = beta;
-      ws.e2[c] = beta * beta;
-      ws.tau[c] = tau;
-      ws.sc[S_TAU] = tau;
-      ws.sc[S_SCALE] = scale;
-    }
+    ws.d[0] = ws.a[0];
+    if (n > 1) reflector_scalars(ws, 0, ws.a[1], s);
   }
 }
 
 // ---------------------------------------------------------------- k_symv
-// y[r] = sum_{s>c} A[r][s] v[s] for rows r > c (one wave per row, 4 rows per
-// wave), with v[c+1] = 1 and v[s] = xtmp[s]*scale.  Writes v into V[j] and into
-// A[c][r] for the block's own rows, and partial dots V_q.v, W_q.v (q < j).
-constexpr int kSymvRows = 16;  // rows per block
+constexpr int kSymvRows = 8;  // rows per block
+constexpr int kSymvNV = 2 * kNB + 2;
 __global__ __launch_bounds__(256) void k_symv(EigWs ws, int32_t n, int32_t c, int32_t j) {
-  __shared__ double vred[kSymvRows][2 * kNB];
+  __shared__ double red[4][kSymvRows];
+  __shared__ double vred[kSymvRows][kSymvNV];
+  __shared__ double mine[kSymvNV], tot[kSymvNV];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int32_t lda = ws.lda;
   const double scale = ws.sc[S_SCALE];
   const int32_t r0 = c + 1 + blockIdx.x * kSymvRows;
-  const double *xt = ws.xtmp;
-  for (int rr = 0; rr < kSymvRows / 4; ++rr) {
-    const int lr = wv * (kSymvRows / 4) + rr;
-    const int32_t r = r0 + lr;
-    if (r >= n) continue;
-    const double *row = ws.a + (int64_t)r * n;
-    double acc = 0.0;
-    for (int32_t s = c + 1 + lane; s < n; s += 64) {
-      double v = (s == c + 1) ? 1.0 : xt[s] * scale;
-      acc += row[s] * v;
-    }
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
-    const double vr = (r == c + 1) ? 1.0 : xt[r] * scale;
-    if (lane == 0) {
-      ws.y[r] = acc;
-      ws.V[(int64_t)j * n + r] = vr;
-      ws.a[(int64_t)c * n + r] = vr;
-    }
-    if (lane < j) {
-      vred[lr][lane] = ws.V[(int64_t)lane * n + r] * vr;
-      vred[lr][kNB + lane] = ws.W[(int64_t)lane * n + r] * vr;
-    }
+  const int32_t a0 = (c + 1) & ~1;
+  const int32_t npair = (((n + 1) & ~1) - a0) >> 1;
+  const double2 *xp = (const double2 *)(ws.xtmp + a0);
+  const double2 *rp[kSymvRows];
+  bool ok[kSymvRows];
+#pragma unroll
+  for (int rr = 0; rr < kSymvRows; ++rr) {
+    ok[rr] = (r0 + rr) < n;
+    rp[rr] = (const double2 *)(ws.a + (int64_t)(ok[rr] ? r0 + rr : r0) * lda + a0);
+  }
+  double acc[kSymvRows];
+#pragma unroll
+  for (int rr = 0; rr < kSymvRows; ++rr) acc[rr] = 0.0;
+  for (int32_t p = threadIdx.x; p < npair; p += 256) {
+    const double2 xx = xp[p];
+    const int32_t s0 = a0 + 2 * p;
+    // v: 0 up to column c, 1 at c+1, x*scale beyond (x is zero past n)
+    const double v0 = (s0 <= c) ? 0.0 : ((s0 == c + 1) ? 1.0 : xx.x * scale);
+    const double v1 = (s0 + 1 == c + 1) ? 1.0 : xx.y * scale;
+    double2 av[kSymvRows];
+#pragma unroll
+    for (int rr = 0; rr < kSymvRows; ++rr) av[rr] = ok[rr] ? rp[rr][p] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int rr = 0; rr < kSymvRows; ++rr) acc[rr] += av[rr].x * v0 + av[rr].y * v1;
+  }
+#pragma unroll
+  for (int rr = 0; rr < kSymvRows; ++rr) {
+    double a = acc[rr];
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+    if (lane == 0) red[wv][rr] = a;
   }
   __syncthreads();
-  if (threadIdx.x < 2 * kNB) {
-    const int q = threadIdx.x & (kNB - 1), isw = threadIdx.x >= kNB;
-    double s = 0.0;
-    if (q < j)
-      for (int lr = 0; lr < kSymvRows; ++lr)
-        if (r0 + lr < n) s += vred[lr][isw * kNB + q];
-    ws.part[(int64_t)blockIdx.x * (2 * kNB) + threadIdx.x] = s;
+  // per row: y, v into V[j] and A[c], and the row's share of p, q, y.v
+  for (int idx = threadIdx.x; idx < kSymvRows * kSymvNV; idx += 256) {
+    const int rr = idx / kSymvNV, q = idx % kSymvNV;
+    const int32_t r = r0 + rr;
+    double val = 0.0;
+    if (r < n) {
+      const double vr = (r == c + 1) ? 1.0 : ws.xtmp[r] * scale;
+      const double yr = (red[0][rr] + red[1][rr]) + (red[2][rr] + red[3][rr]);
+      if (q == 2 * kNB) {
+        val = yr * vr;
+        ws.y[r] = yr;
+        ws.V[(int64_t)j * n + r] = vr;
+        ws.a[(int64_t)c * lda + r] = vr;
+      } else if (q < 2 * kNB && (q & (kNB - 1)) < j) {
+        const double *src = (q < kNB) ? ws.V : ws.W;
+        val = src[(int64_t)(q & (kNB - 1)) * n + r] * vr;
+      }
+    }
+    vred[rr][q] = val;
   }
-  if (!last_block(ws.ticket)) return;
-  if (threadIdx.x < 2 * kNB) {
-    double s = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b) s += ws.part[(int64_t)b * (2 * kNB) + threadIdx.x];
-    const int q = threadIdx.x & (kNB - 1), isw = threadIdx.x >= kNB;
-    ws.sc[(isw ? S_QD : S_PD) + q] = s;
+  __syncthreads();
+  if (threadIdx.x < kSymvNV) {
+    double sum = 0.0;
+#pragma unroll
+    for (int rr = 0; rr < kSymvRows; ++rr) sum += vred[rr][threadIdx.x];
+    mine[threadIdx.x] = sum;
   }
-  if (threadIdx.x == 0) ws.ticket[0] = 0;
+  __syncthreads();
+  if (!hreduce(ws, mine, kSymvNV, tot)) return;
+  if (threadIdx.x < 2 * kNB) ws.sc[S_PD + threadIdx.x] = tot[threadIdx.x];  // pd | qd
+  if (threadIdx.x == 0) ws.sc[S_YV] = tot[2 * kNB];
 }
 
-// ---------------------------------------------------------------- k_wvec
-__global__ __launch_bounds__(256) void k_wvec(EigWs ws, int32_t n, int32_t c, int32_t j) {
+// ---------------------------------------------------------------- k_wcol
+__global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, int32_t j) {
   __shared__ double red[4];
-  const int32_t r = c + 1 + blockIdx.x * 256 + threadIdx.x;
+  __shared__ double pq[2 * kNB];
+  __shared__ double cc[2 * kNB];
+  __shared__ double mine[2], tot[2];
+  const int32_t c1 = c + 1;
+  const int32_t lda = ws.lda;
   const double tau = ws.sc[S_TAU];
-  double part = 0.0;
+  if (threadIdx.x < 2 * kNB) pq[threadIdx.x] = ws.sc[S_PD + threadIdx.x];  // pd | qd
+  if (threadIdx.x < j) {
+    cc[threadIdx.x] = ws.V[(int64_t)threadIdx.x * n + c1];
+    cc[kNB + threadIdx.x] = ws.W[(int64_t)threadIdx.x * n + c1];
+  }
+  __syncthreads();
+  // alpha = -tau/2 w.v with w.v = tau (y.v - 2 sum_q p_q q_q)
+  double spq = 0.0;
+  for (int q = 0; q < j; ++q) spq += pq[q] * pq[kNB + q];
+  const double alpha = -0.5 * tau * (tau * (ws.sc[S_YV] - 2.0 * spq));
+  // w_pre at row c+1 (same in every block)
+  double wc1;
+  {
+    double yy = ws.y[c1];
+    for (int q = 0; q < j; ++q) yy -= cc[q] * pq[kNB + q] + cc[kNB + q] * pq[q];
+    wc1 = tau * yy;
+  }
+  const int32_t r = c1 + blockIdx.x * 256 + threadIdx.x;
+  double nrm = 0.0;
   if (r < n) {
-    double y = ws.y[r];
-    for (int q = 0; q < j; ++q)
-      y -= ws.V[(int64_t)q * n + r] * ws.sc[S_QD + q] + ws.W[(int64_t)q * n + r] * ws.sc[S_PD + q];
-    double w = tau * y;
-    ws.wtmp[r] = w;
-    part = w * ws.V[(int64_t)j * n + r];
+    double yy = ws.y[r];
+    double base = ws.a[(int64_t)c1 * lda + r];
+    for (int q = 0; q < j; ++q) {
+      const double vq = ws.V[(int64_t)q * n + r], wq = ws.W[(int64_t)q * n + r];
+      yy -= vq * pq[kNB + q] + wq * pq[q];
+      base -= cc[q] * wq + cc[kNB + q] * vq;
+    }
+    const double vr = ws.V[(int64_t)j * n + r];
+    const double w = tau * yy;
+    ws.W[(int64_t)j * n + r] = w + alpha * vr;
+    // row c+1 minus panel column j:  V[j][c1] (=1) W[j][r] + W[j][c1] V[j][r]
+    const double x = base - w - wc1 * vr - 2.0 * alpha * vr;
+    ws.xtmp[r] = x;
+    if (r == c1) ws.d[c1] = x;
+    if (r >= c1 + 2) nrm = x * x;
+    // row c1+1 is thread 1 of block 0: its x is the next reflector's alpha and
+    // travels with the sums (the only value the finishing block needs)
+    if (r == c1 + 1) mine[1] = x;
   }
-  double bs = block_sum(part, red);
-  if (threadIdx.x == 0) ws.part[blockIdx.x] = bs;
-  if (!last_block(ws.ticket)) return;
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b) s += ws.part[b];
-    ws.sc[S_ALPHAW] = -0.5 * tau * s;
-    ws.ticket[0] = 0;
-  }
-}
-
-// W[j] = wtmp + alpha V[j] on rows >= r0 (end of panel)
-__global__ void k_wfin(EigWs ws, int32_t n, int32_t r0, int32_t j) {
-  const int32_t r = r0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  ws.W[(int64_t)j * n + r] = ws.wtmp[r] + ws.sc[S_ALPHAW] * ws.V[(int64_t)j * n + r];
+  if (threadIdx.x == 0 && (blockIdx.x != 0 || c1 + 1 >= n)) mine[1] = 0.0;
+  double bs = block_sum(nrm, red);
+  if (threadIdx.x == 0) mine[0] = bs;
+  __syncthreads();
+  if (!hreduce(ws, mine, 2, tot)) return;
+  if (threadIdx.x == 0 && c1 + 1 < n) reflector_scalars(ws, c1, tot[1], tot[0]);
 }
 
 // A[r][s] -= sum_q V[q][r] W[q][s] + W[q][r] V[q][s] for r, s >= k0 (64x64 tiles)
@@ -258,13 +329,13 @@ __global__ __launch_bounds__(256) void k_syr2k(EigWs ws, int32_t n, int32_t k0, 
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       int32_t r = rb + ty + 16 * a, s = cb + tx + 16 * b;
-      if (r < n && s < n) ws.a[(int64_t)r * n + s] -= acc[a][b];
+      if (r < n && s < n) ws.a[(int64_t)r * ws.lda + s] -= acc[a][b];
     }
 }
 
-// ---------------------------------------------------------------- bisection
+// ---------------------------------------------------------------- eigenvalues
 __global__ void k_gersh(EigWs ws, int32_t n) {
-  __shared__ double lo[256], hi[256], mx[256];
+  __shared__ double lo[256], hi[256], mx[256], tnr[256];
   double gl = 1e300, gu = -1e300, emax = 0.0, tn = 0.0;
   for (int32_t i = threadIdx.x; i < n; i += blockDim.x) {
     double r = 0.0;
@@ -277,9 +348,7 @@ __global__ void k_gersh(EigWs ws, int32_t n) {
   }
   lo[threadIdx.x] = gl;
   hi[threadIdx.x] = gu;
-  mx[threadIdx.x] = fmax(emax, 0.0);
-  __syncthreads();
-  __shared__ double tnr[256];
+  mx[threadIdx.x] = emax;
   tnr[threadIdx.x] = tn;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
@@ -302,71 +371,105 @@ __global__ void k_gersh(EigWs ws, int32_t n) {
   }
 }
 
-__device__ inline int32_t sturm(const double *__restrict__ d, const double *__restrict__ e2,
-                                int32_t n, double x, double pivmin) {
-  int32_t cnt = 0;
-  double q = d[0] - x;
-  if (q <= pivmin) {
-    ++cnt;
-    q = fmin(q, -pivmin);
+// Two Sturm counts (eigenvalues of T below x and below x2), interleaved.  The
+// reciprocal is v_rcp_f64 refined by two Newton steps (relative error ~1e-16),
+// which perturbs each pivot far below the bisection tolerance.  d and e2 are read
+// four entries ahead of use.
+__device__ inline void sturm2(const double *__restrict__ d, const double *__restrict__ e2,
+                              int32_t n, double x, double x2, double pivmin, int32_t &c1,
+                              int32_t &c2) {
+  int32_t n1 = 0, n2 = 0;
+  double q = d[0] - x, p = d[0] - x2;
+  if (q <= pivmin) { ++n1; q = fmin(q, -pivmin); }
+  if (p <= pivmin) { ++n2; p = fmin(p, -pivmin); }
+  auto step = [&](double di, double ei) {
+    double r = __builtin_amdgcn_rcp(q), t = __builtin_amdgcn_rcp(p);
+    r = fma(r, fma(-q, r, 1.0), r);
+    t = fma(t, fma(-p, t, 1.0), t);
+    r = fma(r, fma(-q, r, 1.0), r);
+    t = fma(t, fma(-p, t, 1.0), t);
+    q = (di - x) - ei * r;
+    p = (di - x2) - ei * t;
+    if (q <= pivmin) { ++n1; q = fmin(q, -pivmin); }
+    if (p <= pivmin) { ++n2; p = fmin(p, -pivmin); }
+  };
+  int32_t i = 1;
+  for (; i + 3 < n; i += 4) {
+    const double d0 = d[i], d1 = d[i + 1], d2 = d[i + 2], d3 = d[i + 3];
+    const double f0 = e2[i - 1], f1 = e2[i], f2 = e2[i + 1], f3 = e2[i + 2];
+    step(d0, f0);
+    step(d1, f1);
+    step(d2, f2);
+    step(d3, f3);
   }
-  for (int32_t i = 1; i < n; ++i) {
-    q = d[i] - e2[i - 1] / q - x;
-    if (q <= pivmin) {
-      ++cnt;
-      q = fmin(q, -pivmin);
-    }
-  }
-  return cnt;
+  for (; i < n; ++i) step(d[i], e2[i - 1]);
+  c1 = n1;
+  c2 = n2;
 }
 
-// thread k -> k-th smallest eigenvalue
-__global__ void k_bisect(EigWs ws, int32_t n) {
-  int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+// 16 lanes per eigenvalue (k-th smallest): each round the 16 lanes evaluate 32
+// interior points of [lo, hi] and keep the sub-interval holding eigenvalue k.
+constexpr int kMS = 16;
+__global__ __launch_bounds__(256) void k_multisect(EigWs ws, int32_t n) {
+  const int32_t k = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kMS);
+  const int sub = threadIdx.x % kMS;
+  const bool active = k < n;
   double lo = ws.sc[S_GL], hi = ws.sc[S_GU];
   const double pivmin = ws.sc[S_PIVMIN];
   const double eps = 2.220446049250313e-16;
-  for (int it = 0; it < 200; ++it) {
-    double mid = 0.5 * (lo + hi);
-    double tol = 2.0 * eps * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin;
-    if (hi - lo <= tol || mid == lo || mid == hi) break;
-    if (sturm(ws.d, ws.e2, n, mid, pivmin) > k) hi = mid;
-    else lo = mid;
+  for (int it = 0; it < 64; ++it) {
+    const double tol = 2.0 * eps * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin;
+    const bool done = (hi - lo <= tol);
+    // all 16 lanes of a group share lo/hi, so `done` is uniform within a group
+    if (__all(done || !active)) break;
+    const double w = (hi - lo) / (double)(2 * kMS + 1);
+    const double x = lo + w * (double)(2 * sub + 1), x2 = lo + w * (double)(2 * sub + 2);
+    int32_t cnt = 0, cnt2 = 0;
+    if (active && !done) sturm2(ws.d, ws.e2, n, x, x2, pivmin, cnt, cnt2);
+    // new lo = largest point with count <= k ; new hi = smallest point with count > k
+    double cand_lo = (cnt2 <= k) ? x2 : ((cnt <= k) ? x : -1e300);
+    double cand_hi = (cnt > k) ? x : ((cnt2 > k) ? x2 : 1e300);
+    for (int o = kMS / 2; o > 0; o >>= 1) {
+      cand_lo = fmax(cand_lo, __shfl_xor(cand_lo, o, kMS));
+      cand_hi = fmin(cand_hi, __shfl_xor(cand_hi, o, kMS));
+    }
+    if (!done) {
+      double nlo = fmax(lo, cand_lo), nhi = fmin(hi, cand_hi);
+      if (nhi < nlo) nhi = nlo;
+      lo = nlo;
+      hi = nhi;
+    }
   }
-  ws.eval[k] = 0.5 * (lo + hi);
+  if (active && sub == 0) ws.eval[k] = 0.5 * (lo + hi);
 }
 
 // ---------------------------------------------------------------- inverse iteration
-// One block.  Vectors kv = 0..nvec-1 for the eigenvalues eval[n-1-kv] (largest
-// first).  Thread kv factors T - lambda I (LU, partial pivoting) and solves;
-// the block orthogonalises against earlier vectors of the same cluster.
 __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec) {
   __shared__ double red[4];
-  __shared__ double dots[kMaxVec];
+  __shared__ double dots[2];
   const double tnorm = ws.sc[S_TNORM];
   const double eps = 2.220446049250313e-16;
   const double tiny = eps * tnorm;
   const int tid = threadIdx.x;
-  // factor + first solve in parallel, one thread per vector
   if (tid < nvec) {
     const int kv = tid;
     const double lam = ws.eval[n - 1 - kv];
     double *u0 = ws.lu + (int64_t)kv * 5 * n, *u1 = u0 + n, *u2 = u1 + n, *ml = u2 + n,
            *pv = ml + n;
     double acur = ws.d[0] - lam, bcur = (n > 1) ? ws.e[0] : 0.0;
+#pragma unroll 4
     for (int32_t i = 0; i < n - 1; ++i) {
-      double l = ws.e[i];
-      double an = ws.d[i + 1] - lam, bn = (i + 1 < n - 1) ? ws.e[i + 1] : 0.0;
+      const double l = ws.e[i];
+      const double an = ws.d[i + 1] - lam, bn = (i + 1 < n - 1) ? ws.e[i + 1] : 0.0;
       if (fabs(acur) >= fabs(l)) {
         double a0 = acur;
         if (fabs(a0) < tiny) a0 = copysign(tiny, a0 == 0.0 ? 1.0 : a0);
-        double m = l / a0;
+        const double m = l / a0;
         u0[i] = a0; u1[i] = bcur; u2[i] = 0.0; ml[i] = m; pv[i] = 0.0;
         acur = an - m * bcur;
         bcur = bn;
       } else {
-        double m = acur / l;
+        const double m = acur / l;
         u0[i] = l; u1[i] = an; u2[i] = bn; ml[i] = m; pv[i] = 1.0;
         acur = bcur - m * an;
         bcur = -m * bn;
@@ -375,7 +478,6 @@ __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec
     if (fabs(acur) < tiny) acur = copysign(tiny, acur == 0.0 ? 1.0 : acur);
     u0[n - 1] = acur;
     u1[n - 1] = 0.0; u2[n - 1] = 0.0; ml[n - 1] = 0.0; pv[n - 1] = 0.0;
-    // starting vector: deterministic pseudo-random in (0.5, 1.5)
     double *x = ws.X + (int64_t)kv * n;
     for (int32_t i = 0; i < n; ++i) {
       uint32_t h = (uint32_t)(i + 1) * 2654435761u ^ (uint32_t)(kv + 7) * 40503u;
@@ -387,22 +489,31 @@ __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec
   for (int it = 0; it < 3; ++it) {
     if (tid < nvec) {
       const int kv = tid;
-      double *u0 = ws.lu + (int64_t)kv * 5 * n, *u1 = u0 + n, *u2 = u1 + n, *ml = u2 + n,
-             *pv = ml + n;
+      const double *u0 = ws.lu + (int64_t)kv * 5 * n, *u1 = u0 + n, *u2 = u1 + n, *ml = u2 + n,
+                   *pv = ml + n;
       double *x = ws.X + (int64_t)kv * n;
+      double carry = x[0];
+#pragma unroll 4
       for (int32_t i = 0; i < n - 1; ++i) {
+        double nx = x[i + 1];
         if (pv[i] != 0.0) {
-          double t = x[i]; x[i] = x[i + 1]; x[i + 1] = t;
+          const double t = carry; carry = nx; nx = t;
         }
-        x[i + 1] -= ml[i] * x[i];
+        x[i] = carry;
+        carry = nx - ml[i] * carry;
       }
-      x[n - 1] = x[n - 1] / u0[n - 1];
-      if (n > 1) x[n - 2] = (x[n - 2] - u1[n - 2] * x[n - 1]) / u0[n - 2];
-      for (int32_t i = n - 3; i >= 0; --i)
-        x[i] = (x[i] - u1[i] * x[i + 1] - u2[i] * x[i + 2]) / u0[i];
+      x[n - 1] = carry;
+      double x2 = 0.0, x1 = x[n - 1] / u0[n - 1];
+      x[n - 1] = x1;
+#pragma unroll 4
+      for (int32_t i = n - 2; i >= 0; --i) {
+        const double xi = (x[i] - u1[i] * x1 - u2[i] * x2) / u0[i];
+        x[i] = xi;
+        x2 = x1;
+        x1 = xi;
+      }
     }
     __syncthreads();
-    // orthogonalise in order, then normalise
     for (int kv = 0; kv < nvec; ++kv) {
       double *x = ws.X + (int64_t)kv * n;
       const double lam = ws.eval[n - 1 - kv];
@@ -413,9 +524,6 @@ __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec
         double s = 0.0;
         for (int32_t i = tid; i < n; i += 256) s += x[i] * p[i];
         s = block_sum(s, red);
-        if (tid == 0) dots[0] = s;
-        __syncthreads();
-        s = dots[0];
         for (int32_t i = tid; i < n; i += 256) x[i] -= s * p[i];
         __syncthreads();
       }
@@ -432,74 +540,137 @@ __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec
 }
 
 // ---------------------------------------------------------------- back-transform
-// For panel rows c0..c0+nb-1 (Householder vectors in A[c][c+1:], v[c+1] = 1):
-//   Z = V^T X  (nb x nvec) and G = V^T V (nb x nb), reduced by the last block,
-//   which also forms T (forward, columnwise) and Z <- T Z.
-__global__ __launch_bounds__(256) void k_bt_dots(EigWs ws, int32_t n, int32_t c0, int32_t nb,
-                                                 int32_t nvec) {
-  // each block: 256 rows; partial dots for (nb x (nvec + nb)) pairs
-  const int32_t nd = nb * (nvec + nb);
-  const int32_t rbase = c0 + 1 + blockIdx.x * 256;
-  for (int32_t p = threadIdx.x; p < nd; p += 256) {
-    const int q = p / (nvec + nb), o = p % (nvec + nb);
-    const int32_t cq = c0 + q;
-    double s = 0.0;
-    for (int32_t r = rbase; r < min(n, rbase + 256); ++r) {
-      double vq = (r <= cq) ? 0.0 : ws.a[(int64_t)cq * n + r];
-      if (vq == 0.0) continue;
-      double other;
-      if (o < nvec) other = ws.X[(int64_t)o * n + r];
-      else {
-        int32_t co = c0 + (o - nvec);
-        other = (r <= co) ? 0.0 : ws.a[(int64_t)co * n + r];
-      }
-      s += vq * other;
+// One block per panel p (reflector rows c0 = p*kNB .. c0+nb-1, v_q in A[c0+q][c0+q+1:],
+// v_q[c0+q+1] = 1): G = V^T V over all rows, then T (dlarft forward, columnwise:
+// T[0:j, j] = -tau_j T[0:j, 0:j] G[0:j, j]) into Tall[p].
+constexpr int kBtRows = 32;
+__global__ __launch_bounds__(256) void k_bt_prep(EigWs ws, int32_t n, int32_t last) {
+  __shared__ double sv[kNB][kBtRows + 1];
+  __shared__ double G[kNB][kNB + 1], T[kNB][kNB + 1];
+  const int32_t p = blockIdx.x;
+  const int32_t c0 = p * kNB;
+  const int32_t nb = min(kNB, last - c0 + 1);
+  const int32_t lda = ws.lda;
+  // thread owns G entries idx = t + 256 k of the 32x32 grid
+  double g[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int32_t rb = c0 + 1; rb < n; rb += kBtRows) {
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < kNB * kBtRows; idx += 256) {
+      const int q = idx / kBtRows, i = idx % kBtRows;
+      const int32_t r = rb + i, cq = c0 + q;
+      sv[q][i] = (q < nb && r < n && r > cq) ? ws.a[(int64_t)cq * lda + r] : 0.0;
     }
-    ws.part[(int64_t)blockIdx.x * nd + p] = s;
-  }
-  if (!last_block(ws.ticket)) return;
-  __shared__ double G[kNB][kNB], T[kNB][kNB], Zs[kNB][kMaxVec];
-  for (int32_t p = threadIdx.x; p < nd; p += 256) {
-    double s = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b) s += ws.part[(int64_t)b * nd + p];
-    const int q = p / (nvec + nb), o = p % (nvec + nb);
-    if (o < nvec) Zs[q][o] = s;
-    else G[q][o - nvec] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    ws.ticket[0] = 0;
-    // dlarft forward/columnwise: T[j][j] = tau_j; T[0:j, j] = -tau_j T[0:j,0:j] G[0:j, j]
-    for (int jj = 0; jj < nb; ++jj) {
-      const double tj = ws.tau[c0 + jj];
-      for (int i = 0; i < jj; ++i) {
-        double s = 0.0;
-        for (int k = i; k < jj; ++k) s += T[i][k] * G[k][jj];
-        T[i][jj] = -tj * s;
-      }
-      T[jj][jj] = tj;
-      for (int i = jj + 1; i < nb; ++i) T[i][jj] = 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int idx = threadIdx.x + 256 * k;
+      const int q1 = idx / kNB, q2 = idx % kNB;
+      double s = 0.0;
+      for (int i = 0; i < kBtRows; ++i) s += sv[q1][i] * sv[q2][i];
+      g[k] += s;
     }
   }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int idx = threadIdx.x + 256 * k;
+    G[idx / kNB][idx % kNB] = g[k];
+  }
+  for (int idx = threadIdx.x; idx < kNB * kNB; idx += 256) T[idx / kNB][idx % kNB] = 0.0;
   __syncthreads();
-  // Z <- T Z
-  for (int32_t p = threadIdx.x; p < nb * nvec; p += 256) {
-    const int q = p / nvec, o = p % nvec;
+  for (int jj = 0; jj < nb; ++jj) {
+    const double tj = ws.tau[c0 + jj];
+    // T[i][jj] for i < jj in parallel; T is upper triangular
     double s = 0.0;
-    for (int k = q; k < nb; ++k) s += T[q][k] * Zs[k][o];
+    const int i = threadIdx.x;
+    if (i < jj)
+      for (int k = i; k < jj; ++k) s += T[i][k] * G[k][jj];
+    __syncthreads();
+    if (i < jj) T[i][jj] = -tj * s;
+    if (i == jj) T[jj][jj] = tj;
+    __syncthreads();
+  }
+  for (int idx = threadIdx.x; idx < kNB * kNB; idx += 256)
+    ws.Tall[(int64_t)p * kNB * kNB + idx] = T[idx / kNB][idx % kNB];
+}
+
+// Fused step for panel p: X[r] -= sum_q v_{p+1,q}[r] Z[q] (previous panel, if any),
+// then partial dots V_p^T X over this block's rows; the finishing block forms
+// Z <- T_p (V_p^T X) for the next step.
+__global__ __launch_bounds__(256) void k_bt_step(EigWs ws, int32_t n, int32_t p, int32_t last,
+                                                 int32_t nvec, int32_t apply_prev) {
+  // sz (previous panel's Z) is dead once applied, so the dot partials reuse it
+  __shared__ double szbuf[kNB * kMaxVec];
+  __shared__ double sv[kNB][kBtRows + 1];
+  __shared__ double sx[kMaxVec][kBtRows + 1];
+  double(*sz)[kMaxVec] = (double(*)[kMaxVec])szbuf;
+  double *mine = szbuf, *tot = szbuf;
+  const int32_t lda = ws.lda;
+  const int32_t c0 = p * kNB;
+  const int32_t nb = min(kNB, last - c0 + 1);
+  const int32_t rbase = c0 + 1 + blockIdx.x * kBtRows;
+  const int32_t rows = min(kBtRows, n - rbase);
+  if (apply_prev) {
+    for (int idx = threadIdx.x; idx < kNB * nvec; idx += 256)
+      sz[idx / nvec][idx % nvec] = ws.Z[(idx / nvec) * kMaxVec + (idx % nvec)];
+  }
+  for (int idx = threadIdx.x; idx < nvec * kBtRows; idx += 256) {
+    const int o = idx / kBtRows, i = idx % kBtRows;
+    sx[o][i] = (i < rows) ? ws.X[(int64_t)o * n + rbase + i] : 0.0;
+  }
+  __syncthreads();
+  if (apply_prev) {
+    const int32_t c0n = c0 + kNB;  // previous (higher) panel
+    const int32_t nbn = min(kNB, last - c0n + 1);
+    for (int idx = threadIdx.x; idx < nvec * kBtRows; idx += 256) {
+      const int o = idx / kBtRows, i = idx % kBtRows;
+      const int32_t r = rbase + i;
+      if (i >= rows) continue;
+      double s = 0.0;
+      for (int q = 0; q < nbn; ++q) {
+        const int32_t cq = c0n + q;
+        if (r > cq) s += ws.a[(int64_t)cq * lda + r] * sz[q][o];
+      }
+      sx[o][i] -= s;
+      ws.X[(int64_t)o * n + r] = sx[o][i];
+    }
+  }
+  for (int idx = threadIdx.x; idx < kNB * kBtRows; idx += 256) {
+    const int q = idx / kBtRows, i = idx % kBtRows;
+    const int32_t r = rbase + i, cq = c0 + q;
+    sv[q][i] = (q < nb && i < rows && r > cq) ? ws.a[(int64_t)cq * lda + r] : 0.0;
+  }
+  __syncthreads();  // sz consumed, sv and sx complete
+  const int nd = kNB * nvec;
+  for (int idx = threadIdx.x; idx < nd; idx += 256) {
+    const int q = idx / nvec, o = idx % nvec;
+    double s = 0.0;
+    for (int i = 0; i < rows; ++i) s += sv[q][i] * sx[o][i];
+    mine[idx] = s;
+  }
+  __syncthreads();
+  if (!hreduce(ws, mine, nd, tot)) return;
+  const double *T = ws.Tall + (int64_t)p * kNB * kNB;
+  for (int idx = threadIdx.x; idx < nd; idx += 256) {
+    const int q = idx / nvec, o = idx % nvec;
+    double s = 0.0;
+    for (int k = q; k < nb; ++k) s += T[q * kNB + k] * tot[k * nvec + o];
     ws.Z[q * kMaxVec + o] = s;
   }
 }
 
-// X[o][r] -= sum_q v_q[r] Z[q][o]
-__global__ void k_bt_apply(EigWs ws, int32_t n, int32_t c0, int32_t nb, int32_t nvec) {
-  const int32_t r = c0 + 1 + blockIdx.x * blockDim.x + threadIdx.x;
+// final application of panel 0's Z
+__global__ __launch_bounds__(256) void k_bt_apply0(EigWs ws, int32_t n, int32_t last,
+                                                   int32_t nvec) {
+  const int32_t r = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
+  const int32_t nb = min(kNB, last + 1);
   double v[kNB];
-  for (int q = 0; q < nb; ++q) v[q] = (r <= c0 + q) ? 0.0 : ws.a[(int64_t)(c0 + q) * n + r];
+#pragma unroll
+  for (int q = 0; q < kNB; ++q) v[q] = (q < nb && r > q) ? ws.a[(int64_t)q * ws.lda + r] : 0.0;
   for (int o = 0; o < nvec; ++o) {
     double s = 0.0;
-    for (int q = 0; q < nb; ++q) s += v[q] * ws.Z[q * kMaxVec + o];
+#pragma unroll
+    for (int q = 0; q < kNB; ++q) s += v[q] * ws.Z[q * kMaxVec + o];
     ws.X[(int64_t)o * n + r] -= s;
   }
 }
@@ -509,6 +680,7 @@ __global__ __launch_bounds__(256) void k_finish_vec(EigWs ws, int32_t n) {
   __shared__ double red[4];
   __shared__ double bv[256];
   __shared__ int32_t bi[256];
+  __shared__ double fac;
   double *x = ws.X + (int64_t)blockIdx.x * n;
   double s = 0.0, best = -1.0;
   int32_t besti = 0x7fffffff;
@@ -535,7 +707,6 @@ __global__ __launch_bounds__(256) void k_finish_vec(EigWs ws, int32_t n) {
     }
     __syncthreads();
   }
-  __shared__ double fac;
   if (threadIdx.x == 0) {
     double f = 1.0 / sqrt(s);
     if (x[bi[0]] < 0.0) f = -f;
@@ -545,31 +716,61 @@ __global__ __launch_bounds__(256) void k_finish_vec(EigWs ws, int32_t n) {
   for (int32_t i = threadIdx.x; i < n; i += 256) x[i] *= fac;
 }
 
+// ---------------------------------------------------------------- driver
+static void launch_tridiag(EigWs ws, int32_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_col0, dim3(1), dim3(256), 0, st, ws, n);
+  for (int32_t c = 0; c + 1 < n; ++c) {
+    const int32_t j = c % kNB;
+    const int32_t L = n - c - 1;
+    unsigned gsym = (unsigned)((L + kSymvRows - 1) / kSymvRows);
+    hipLaunchKernelGGL(k_symv, dim3(gsym), dim3(256), 0, st, ws, n, c, j);
+    unsigned gw = (unsigned)((L + 255) / 256);
+    hipLaunchKernelGGL(k_wcol, dim3(gw), dim3(256), 0, st, ws, n, c, j);
+    // panel end: update the trailing matrix before the next panel's first symv
+    if (j == kNB - 1 && c + 2 < n) {
+      const int32_t k1 = c + 1;
+      int32_t nt = (n - k1 + 63) / 64;
+      hipLaunchKernelGGL(k_syr2k, dim3((unsigned)(nt * nt)), dim3(256), 0, st, ws, n, k1, kNB);
+    }
+  }
+}
+
+struct TridiagGraph {
+  int32_t n = -1;
+  void *a = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+static TridiagGraph g_graph[16];  // per device
+
 int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, double *evals,
                  double *evecs) {
   if (n <= 0) return eig_set_err(ctx, EIG_E_ARG, "eigh: n <= 0");
   if (nvec < 0 || nvec > kMaxVec || nvec > n)
     return eig_set_err(ctx, EIG_E_ARG, "eigh: nvec must be in [0, min(n, %d)]", kMaxVec);
   const int64_t N = n;
-  const int nblk_max = (int)((N + 255) / 256) + 8;
-  const int nsym_max = (int)((N + kSymvRows - 1) / kSymvRows) + 8;
-  size_t part_n = std::max<size_t>((size_t)nsym_max * 2 * kNB,
-                                   (size_t)nblk_max * kNB * (kMaxVec + kNB));
+  const int32_t lda = (int32_t)(((N + 7) / 8) * 8);
+  const int64_t maxblk = (N + kSymvRows - 1) / kSymvRows + 8;
+  const int64_t maxgrp = (maxblk + kGroup - 1) / kGroup + 2;
+  const int64_t maxnv = std::max<int64_t>(kSymvNV, (int64_t)kNB * kMaxVec);
+  const int64_t npanel = (N + kNB - 1) / kNB + 1;
   size_t need = 0;
   auto add = [&](size_t cnt) {
     size_t o = need;
     need += ((cnt * 8 + 255) / 256) * 256;
     return o;
   };
-  size_t o_a = add(N * N), o_d = add(N), o_e = add(N), o_e2 = add(N), o_tau = add(N),
-         o_V = add(kNB * N), o_W = add(kNB * N), o_x = add(N), o_w = add(N), o_y = add(N),
-         o_sc = add(S_COUNT), o_part = add(part_n), o_tk = add(8), o_ev = add(N),
-         o_X = add((size_t)std::max(nvec, 1) * N), o_lu = add((size_t)std::max(nvec, 1) * 5 * N),
-         o_Z = add(kNB * kMaxVec);
+  size_t o_a = add((size_t)N * lda), o_d = add(N), o_e = add(N), o_e2 = add(N),
+         o_tau = add(N), o_V = add(kNB * N), o_W = add(kNB * N), o_x = add(lda), o_y = add(N),
+         o_sc = add(S_COUNT), o_p1 = add((size_t)maxblk * maxnv),
+         o_p2 = add((size_t)maxgrp * maxnv), o_tk = add((size_t)maxgrp + 8), o_ev = add(N),
+         o_X = add((size_t)kMaxVec * N), o_lu = add((size_t)kMaxVec * 5 * N),
+         o_Z = add(kNB * kMaxVec), o_T = add((size_t)npanel * kNB * kNB);
+  void *old = ctx->ework.p;
   EIG_TRY(eig_ensure(ctx, ctx->ework, need));
   char *base = (char *)ctx->ework.p;
   EigWs ws;
   ws.a = (double *)(base + o_a);
+  ws.lda = lda;
   ws.d = (double *)(base + o_d);
   ws.e = (double *)(base + o_e);
   ws.e2 = (double *)(base + o_e2);
@@ -577,75 +778,87 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
   ws.V = (double *)(base + o_V);
   ws.W = (double *)(base + o_W);
   ws.xtmp = (double *)(base + o_x);
-  ws.wtmp = (double *)(base + o_w);
   ws.y = (double *)(base + o_y);
   ws.sc = (double *)(base + o_sc);
-  ws.part = (double *)(base + o_part);
+  ws.part1 = (double *)(base + o_p1);
+  ws.part2 = (double *)(base + o_p2);
   ws.ticket = (unsigned *)(base + o_tk);
   ws.eval = (double *)(base + o_ev);
   ws.X = (double *)(base + o_X);
   ws.lu = (double *)(base + o_lu);
   ws.Z = (double *)(base + o_Z);
+  ws.Tall = (double *)(base + o_T);
   hipStream_t st = ctx->stream;
-  EIG_HIP(ctx, hipMemcpyAsync(ws.a, d_a, sizeof(double) * N * N, hipMemcpyDeviceToDevice, st));
-  EIG_HIP(ctx, hipMemsetAsync(ws.ticket, 0, 64, st));
+  if (lda != n) EIG_HIP(ctx, hipMemsetAsync(ws.a, 0, sizeof(double) * N * lda, st));
+  EIG_HIP(ctx, hipMemcpy2DAsync(ws.a, sizeof(double) * lda, d_a, sizeof(double) * N,
+                                sizeof(double) * N, N, hipMemcpyDeviceToDevice, st));
+  EIG_HIP(ctx, hipMemsetAsync(ws.ticket, 0, sizeof(unsigned) * (maxgrp + 8), st));
   EIG_HIP(ctx, hipMemsetAsync(ws.sc, 0, sizeof(double) * S_COUNT, st));
   EIG_HIP(ctx, hipMemsetAsync(ws.e, 0, sizeof(double) * N, st));
   EIG_HIP(ctx, hipMemsetAsync(ws.e2, 0, sizeof(double) * N, st));
   EIG_HIP(ctx, hipMemsetAsync(ws.tau, 0, sizeof(double) * N, st));
+  EIG_HIP(ctx, hipMemsetAsync(ws.V, 0, sizeof(double) * kNB * N, st));
+  EIG_HIP(ctx, hipMemsetAsync(ws.W, 0, sizeof(double) * kNB * N, st));
+  EIG_HIP(ctx, hipMemsetAsync(ws.xtmp, 0, sizeof(double) * lda, st));
 
+  // ---- tridiagonalisation (graph replay for n above a few hundred)
   eig_time_begin(ctx);
-  // ---- tridiagonalisation
-  for (int32_t k0 = 0; k0 < n; k0 += kNB) {
-    const int32_t nb = std::min<int32_t>(kNB, n - k0);
-    for (int32_t j = 0; j < nb; ++j) {
-      const int32_t c = k0 + j;
-      unsigned gcol = (unsigned)((n - c + 255) / 256);
-      hipLaunchKernelGGL(k_col, dim3(gcol), dim3(256), 0, st, ws, n, c, j);
-      if (c + 1 >= n) continue;  // last row: only d
-      unsigned gsym = (unsigned)((n - c - 1 + kSymvRows - 1) / kSymvRows);
-      hipLaunchKernelGGL(k_symv, dim3(gsym), dim3(256), 0, st, ws, n, c, j);
-      unsigned gw = (unsigned)((n - c - 1 + 255) / 256);
-      hipLaunchKernelGGL(k_wvec, dim3(gw), dim3(256), 0, st, ws, n, c, j);
+  const bool use_graph = n >= 256 && getenv("EIG_NO_GRAPH") == nullptr;
+  if (use_graph) {
+    TridiagGraph &G = g_graph[ctx->device & 15];
+    if (G.exec == nullptr || G.n != n || G.a != (void *)ws.a || old != ctx->ework.p) {
+      if (G.exec) hipGraphExecDestroy(G.exec);
+      G.exec = nullptr;
+      hipGraph_t graph;
+      EIG_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      launch_tridiag(ws, n, st);
+      EIG_HIP(ctx, hipStreamEndCapture(st, &graph));
+      EIG_HIP(ctx, hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0));
+      hipGraphDestroy(graph);
+      G.n = n;
+      G.a = ws.a;
     }
-    const int32_t k1 = k0 + nb;
-    if (k1 < n) {
-      const int32_t clast = k0 + nb - 1;
-      unsigned gf = (unsigned)((n - clast - 1 + 255) / 256);
-      hipLaunchKernelGGL(k_wfin, dim3(gf), dim3(256), 0, st, ws, n, clast + 1, nb - 1);
-      int32_t nt = (n - k1 + 63) / 64;
-      hipLaunchKernelGGL(k_syr2k, dim3((unsigned)(nt * nt)), dim3(256), 0, st, ws, n, k1, nb);
-    }
+    EIG_HIP(ctx, hipGraphLaunch(G.exec, st));
+  } else {
+    launch_tridiag(ws, n, st);
   }
   EIG_HIP(ctx, hipGetLastError());
   eig_time_end(ctx, "tridiag");
   // ---- eigenvalues
   eig_time_begin(ctx);
   hipLaunchKernelGGL(k_gersh, dim3(1), dim3(256), 0, st, ws, n);
-  hipLaunchKernelGGL(k_bisect, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st, ws, n);
+  {
+    int64_t threads = (int64_t)n * kMS;
+    hipLaunchKernelGGL(k_multisect, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, ws, n);
+  }
   EIG_HIP(ctx, hipGetLastError());
   eig_time_end(ctx, "bisect");
   // ---- eigenvectors
   eig_time_begin(ctx);
   if (nvec > 0) {
     hipLaunchKernelGGL(k_invit, dim3(1), dim3(256), 0, st, ws, n, nvec);
-    // back-transform, last panel first; panel columns c with c+1 < n carry reflectors
-    int32_t last = n - 2;  // last reflector row
+    eig_time_end(ctx, "invit");
+    eig_time_begin(ctx);
+    const int32_t last = n - 2;  // last reflector row
     if (last >= 0) {
-      int32_t p_last = last / kNB;
+      const int32_t p_last = last / kNB;
+      hipLaunchKernelGGL(k_bt_prep, dim3((unsigned)(p_last + 1)), dim3(256), 0, st, ws, n, last);
       for (int32_t p = p_last; p >= 0; --p) {
         const int32_t c0 = p * kNB;
-        const int32_t nb = std::min<int32_t>(kNB, last - c0 + 1);
-        unsigned g = (unsigned)((n - c0 - 1 + 255) / 256);
-        hipLaunchKernelGGL(k_bt_dots, dim3(g), dim3(256), 0, st, ws, n, c0, nb, nvec);
-        hipLaunchKernelGGL(k_bt_apply, dim3(g), dim3(256), 0, st, ws, n, c0, nb, nvec);
+        const unsigned g = (unsigned)((n - c0 - 1 + kBtRows - 1) / kBtRows);
+        hipLaunchKernelGGL(k_bt_step, dim3(g), dim3(256), 0, st, ws, n, p, last, nvec,
+                           (int32_t)(p < p_last));
       }
+      hipLaunchKernelGGL(k_bt_apply0, dim3((unsigned)((n - 1 + 255) / 256)), dim3(256), 0, st, ws,
+                         n, last, nvec);
     }
     hipLaunchKernelGGL(k_finish_vec, dim3((unsigned)nvec), dim3(256), 0, st, ws, n);
     EIG_HIP(ctx, hipGetLastError());
+  } else {
+    eig_time_end(ctx, "invit");
+    eig_time_begin(ctx);
   }
   eig_time_end(ctx, "eigvec");
-  // ---- results: descending eigenvalues
   std::vector<double> asc(N);
   EIG_HIP(ctx, hipMemcpyAsync(asc.data(), ws.eval, sizeof(double) * N, hipMemcpyDeviceToHost, st));
   if (nvec > 0 && evecs)

@@ -10,6 +10,10 @@
 //  form BF16X3 : each level split into bf16 hi + lo; hi*hi + hi*lo + lo*hi on
 //                v_mfma_f32_32x32x16_bf16 into FP32 accumulators, which are
 //                added into FP64 accumulators every `blk` SNPs.
+//  form BF16X4 : as BF16X3 plus lo*lo.  The dropped lo*lo term is not noise: every
+//                pair of samples sharing a genotype at a SNP loses lo^2 > 0, a
+//                systematic bias (~3e-6 of the diagonal) that dominates the error
+//                of the top eigenvalues; the fourth product removes it.
 #include <cmath>
 #include <vector>
 
@@ -137,6 +141,7 @@ __global__ __launch_bounds__(256, 2) void k_syrk_f64(const uint32_t *__restrict_
 constexpr int kKT16 = 16;      // SNPs per LDS stage (one MFMA k-step)
 constexpr int kRowB = 24;      // bf16 per LDS row: 16 + 8 pad -> 48 B stride
 
+template <bool kLoLo>
 __global__ __launch_bounds__(256, 2) void k_syrk_bf16x3(
     const uint32_t *__restrict__ sel, int64_t M, const uint32_t *__restrict__ levbf, int32_t P,
     int64_t chunk, int32_t blk, double *__restrict__ part) {
@@ -250,6 +255,8 @@ __global__ __launch_bounds__(256, 2) void k_syrk_bf16x3(
       for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
+          if (kLoLo)
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bl[b], acc[a][b], 0, 0, 0);
           acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
           acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
           acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
@@ -330,13 +337,14 @@ int eig_run_syrk(eig_ctx *ctx, int form, int bf16_block, double *trace_scale) {
   const int32_t T = ctx->nst;
   const int32_t P = T * (T + 1) / 2;
   const int32_t n = ctx->nrows;
-  if (form == EIG_FORM_BF16X3) {
+  const bool split = (form == EIG_FORM_BF16X3 || form == EIG_FORM_BF16X4);
+  if (split) {
     if (bf16_block <= 0) bf16_block = 4096;
     if (bf16_block % kKT16 != 0) return eig_set_err(ctx, EIG_E_ARG, "bf16_block % 16 != 0");
   }
   // split the SNP range so that pairs*splits fills the chip a few times over;
   // chunks are multiples of the FP32 block (bf16) or of 256 SNPs.
-  const int64_t align = (form == EIG_FORM_BF16X3) ? bf16_block : 256;
+  const int64_t align = split ? bf16_block : 256;
   int64_t want = std::max<int64_t>(1, (4LL * ctx->num_cu + P - 1) / P);
   int64_t chunk = (M + want - 1) / want;
   chunk = ((chunk + align - 1) / align) * align;
@@ -351,8 +359,12 @@ int eig_run_syrk(eig_ctx *ctx, int form, int bf16_block, double *trace_scale) {
     hipLaunchKernelGGL(k_syrk_f64, dim3(grid), dim3(256), 0, ctx->stream,
                        (const uint32_t *)ctx->sel.p, M, (const double *)ctx->lev.p, P, chunk,
                        (double *)ctx->part.p);
+  } else if (form == EIG_FORM_BF16X3) {
+    hipLaunchKernelGGL(k_syrk_bf16x3<false>, dim3(grid), dim3(256), 0, ctx->stream,
+                       (const uint32_t *)ctx->sel.p, M, (const uint32_t *)ctx->levbf.p, P,
+                       chunk, bf16_block, (double *)ctx->part.p);
   } else {
-    hipLaunchKernelGGL(k_syrk_bf16x3, dim3(grid), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_syrk_bf16x3<true>, dim3(grid), dim3(256), 0, ctx->stream,
                        (const uint32_t *)ctx->sel.p, M, (const uint32_t *)ctx->levbf.p, P,
                        chunk, bf16_block, (double *)ctx->part.p);
   }
@@ -380,7 +392,7 @@ int eig_run_syrk(eig_ctx *ctx, int form, int bf16_block, double *trace_scale) {
 extern "C" int eig_xtx(eig_ctx *ctx, int32_t form, int32_t bf16_block, double *trace_scale,
                        double *xtx_out) {
   if (!ctx) return EIG_E_ARG;
-  if (form != EIG_FORM_FP64 && form != EIG_FORM_BF16X3)
+  if (form != EIG_FORM_FP64 && form != EIG_FORM_BF16X3 && form != EIG_FORM_BF16X4)
     return eig_set_err(ctx, EIG_E_ARG, "unknown XTX form %d", form);
   if (!ctx->have_stats) return eig_set_err(ctx, EIG_E_STATE, "eig_xtx before eig_snpstats");
   EIG_HIP(ctx, hipSetDevice(ctx->device));

@@ -31,8 +31,9 @@ extern "C" {
 
 /* XTX accumulation forms */
 #define EIG_FORM_FP64 0   /* FP64 end to end (MFMA f64)                             */
-#define EIG_FORM_BF16X3 1 /* split-BF16 hi+lo, 3 MFMA products, FP32 per SNP block,
-                             FP64 across blocks                                     */
+#define EIG_FORM_BF16X3 1 /* split-BF16 hi+lo, 3 MFMA products (hi.hi, hi.lo, lo.hi),
+                             FP32 per SNP block, FP64 across blocks                 */
+#define EIG_FORM_BF16X4 2 /* as BF16X3 plus the lo.lo product                       */
 
 /* per-SNP status written by eig_snpstats */
 #define EIG_SNP_KEPT 0
@@ -89,7 +90,7 @@ int eig_snpstats(eig_ctx *ctx, const uint8_t *live, int32_t altnorm, int32_t fan
  * (smartpca.c:3426-3495, 480-508, 1222-1236).  The normalised matrix stays on the
  * device for eig_eigh_xtx; xtx_out (nrows*nrows, row-major) may be NULL.
  * *trace_scale receives trace/(nrows-1) before scaling.  bf16_block is the FP32
- * accumulation length in SNPs for EIG_FORM_BF16X3 (multiple of 64; 0 = 4096). */
+ * accumulation length in SNPs for the split-BF16 forms (multiple of 16; 0 = 4096). */
 int eig_xtx(eig_ctx *ctx, int32_t form, int32_t bf16_block, double *trace_scale,
             double *xtx_out);
 
