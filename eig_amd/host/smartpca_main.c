@@ -1,0 +1,587 @@
+/*
+ * smartpca_main.c -- `smartpca -p parfile`, the drop-in for EIGENSOFT's smartpca
+ * (src/eigensrc/smartpca.c) on one MI355X.
+ *
+ * Same parfile keys, inputs and outputs as the reference (readcommands,
+ * smartpca.c:2055-2233; main, smartpca.c:537-1591).  Host code does what the
+ * reference's main does around its hot loops -- reading, filtering, the outlier
+ * decisions, formatting -- and every pass over genotypes or the N x N matrix
+ * runs on the GPU through the C-ABI in eig_abi.h: column statistics, XTX,
+ * eigensolve, loadings, projection scales and lsqproject.
+ *
+ * Extra keys (absent from the reference, ignored by it):
+ *   xtxform:  fp64 (default) | bf16x4 | bf16x3   XTX accumulation form
+ *   bf16block: FP32 accumulation length in SNPs for the split-BF16 forms (256)
+ *   gpu:      HIP device index (0)
+ */
+#include <ctype.h>
+#include <getopt.h>
+#include <math.h>
+#include <time.h>
+#include <unistd.h>
+
+#include "smartpca.h"
+
+void die(const char *fmt, ...) {
+  va_list ap;
+  fflush(stdout);
+  fprintf(stderr, "fatalx:\n");
+  va_start(ap, fmt);
+  vfprintf(stderr, fmt, ap);
+  va_end(ap);
+  fprintf(stderr, "\n");
+  exit(1);
+}
+
+#define CHK(call)                                                            \
+  do {                                                                       \
+    int rc_ = (call);                                                        \
+    if (rc_ != EIG_OK) die("%s failed (%d): %s", #call, rc_, eig_last_error(ctx)); \
+  } while (0)
+
+typedef struct {
+  char *genotypename, *snpname, *indivname, *poplistname, *outputname, *outputvname;
+  char *outliername, *snpeigname, *grmoutname, *deletesnpoutname, *xtxform;
+  int numeigs, numoutliter, numoutleigs, outliermode, altnormstyle, fancynorm;
+  int minallelecnt, maxmissing, noxdata, numchrom, hiprec, regmode, hashcheck;
+  int familynames, doproject, malexhet, grmbinary, xchrom, zchrom, lopos, hipos;
+  int pordercheck, fastmode, fastdim, fastiter, bf16block, gpu, verbose;
+  double outlthresh;
+  long seed;
+} Settings;
+
+static void defaults(Settings *s) {
+  memset(s, 0, sizeof *s);
+  s->numeigs = 10;            /* smartpca.c:164 */
+  s->numoutliter = 5;         /* smartpca.c:211 */
+  s->numoutleigs = 10;
+  s->outlthresh = 6.0;        /* smartpca.c:212 */
+  s->altnormstyle = 1;        /* smartpca.c:186 */
+  s->fancynorm = 1;           /* include/globals.h:5 */
+  s->minallelecnt = 1;        /* smartpca.c:187 */
+  s->maxmissing = 9999999;    /* smartpca.c:188 */
+  s->noxdata = 1;             /* smartpca.c:178 */
+  s->numchrom = 22;           /* include/globals.h:4 */
+  s->regmode = 1;             /* smartpca.c:208 */
+  s->hashcheck = 1;
+  s->familynames = 1;
+  s->doproject = 1;
+  s->xchrom = -1;
+  s->zchrom = -1;
+  s->lopos = -999999999;
+  s->hipos = 999999999;
+  s->pordercheck = 1;
+  s->fastdim = -1;
+  s->fastiter = -1;
+  s->bf16block = 256;
+}
+
+static void read_settings(const Params *pp, Settings *s) {
+  par_string(pp, "genotypename:", &s->genotypename);
+  par_string(pp, "snpname:", &s->snpname);
+  par_string(pp, "indivname:", &s->indivname);
+  par_string(pp, "poplistname:", &s->poplistname);
+  par_string(pp, "snpeigname:", &s->snpeigname);
+  par_string(pp, "snpweightoutname:", &s->snpeigname);
+  par_string(pp, "output:", &s->outputname);
+  par_string(pp, "outputvecs:", &s->outputname);
+  par_string(pp, "evecoutname:", &s->outputname);
+  par_string(pp, "outputvals:", &s->outputvname);
+  par_string(pp, "evaloutname:", &s->outputvname);
+  par_string(pp, "outliername:", &s->outliername);
+  par_string(pp, "outlieroutname:", &s->outliername);
+  par_string(pp, "deletsnpoutname:", &s->deletesnpoutname);
+  par_string(pp, "grmoutname:", &s->grmoutname);
+  par_int(pp, "grmbinary:", &s->grmbinary);
+  par_int(pp, "numeigs:", &s->numeigs);
+  par_int(pp, "numoutevec:", &s->numeigs);
+  par_int(pp, "fancynorm:", &s->fancynorm);
+  par_int(pp, "usenorm:", &s->fancynorm);
+  par_int(pp, "altnormstyle:", &s->altnormstyle);
+  par_int(pp, "popgenmode:", &s->altnormstyle);
+  par_int(pp, "hashcheck:", &s->hashcheck);
+  par_int(pp, "noxdata:", &s->noxdata);
+  int t = -1;
+  par_int(pp, "xdata:", &t);
+  if (t >= 0) s->noxdata = 1 - t;
+  par_int(pp, "fastmode:", &s->fastmode);
+  par_int(pp, "fastdim:", &s->fastdim);
+  par_int(pp, "fastiter:", &s->fastiter);
+  t = 0;
+  par_int(pp, "seed:", &t);
+  s->seed = t;
+  par_int(pp, "regmode:", &s->regmode);
+  par_int(pp, "lsqproject:", &s->regmode);
+  par_int(pp, "doproject:", &s->doproject);
+  par_int(pp, "minallelecnt:", &s->minallelecnt);
+  par_int(pp, "chrom:", &s->xchrom);
+  par_int(pp, "nochrom:", &s->zchrom);
+  par_int(pp, "maxmissing:", &s->maxmissing);
+  par_int(pp, "lopos:", &s->lopos);
+  par_int(pp, "hipos:", &s->hipos);
+  par_int(pp, "hiprecision:", &s->hiprec);
+  par_int(pp, "hiprec:", &s->hiprec);
+  par_int(pp, "malexhet:", &s->malexhet);
+  par_int(pp, "nomalexhet:", &s->malexhet);
+  par_int(pp, "familynames:", &s->familynames);
+  par_int(pp, "numoutliter:", &s->numoutliter);
+  par_int(pp, "numoutlieriter:", &s->numoutliter);
+  par_int(pp, "numoutleigs", &s->numoutleigs); /* sic: no colon in the reference */
+  par_int(pp, "numoutlierevec:", &s->numoutleigs);
+  par_dbl(pp, "outlthresh:", &s->outlthresh);
+  par_dbl(pp, "outliersigmathresh:", &s->outlthresh);
+  par_int(pp, "outliermode:", &s->outliermode);
+  par_int(pp, "numchrom:", &s->numchrom);
+  par_int(pp, "pordercheck:", &s->pordercheck);
+  par_string(pp, "xtxform:", &s->xtxform);
+  par_int(pp, "bf16block:", &s->bf16block);
+  par_int(pp, "gpu:", &s->gpu);
+}
+
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+static void log_deleted(const Settings *s, const char *id, const char *why) {
+  if (!s->deletesnpoutname) return;
+  FILE *f = fopen(s->deletesnpoutname, "a");
+  if (!f) return;
+  fprintf(f, "%20s %s\n", id, why);
+  fclose(f);
+}
+
+/* ridoutlier (smartsubs.c:18-93) */
+static int ridoutlier(const double *evecs, int n, int neigs, double thresh, int mode, int *bad,
+                      int *vecno, double *score) {
+  if (mode > 1 || n < 3) return 0;
+  int *flag = calloc(n, sizeof(int)), *vno = malloc(sizeof(int) * n);
+  double *sc = calloc(n, sizeof(double)), *w = malloc(sizeof(double) * n),
+         *w2 = malloc(sizeof(double) * n);
+  for (int j = 0; j < n; ++j) vno[j] = -1;
+  for (int e = 0; e < neigs; ++e) {
+    memcpy(w, evecs + (int64_t)e * n, sizeof(double) * n);
+    if (mode == 0) {
+      double s = 0.0;
+      for (int j = 0; j < n; ++j) s += w[j];
+      double mu = s / (double)n, s2 = 0.0;
+      for (int j = 0; j < n; ++j) w[j] += -mu;
+      for (int j = 0; j < n; ++j) s2 += w[j] * w[j];
+      double r = 1.0 / sqrt(s2 / (double)n);
+      for (int j = 0; j < n; ++j) {
+        w[j] *= r;
+        if (fabs(w[j]) > thresh) {
+          flag[j] = 1;
+          if (vno[j] < 0) { vno[j] = e; sc[j] = w[j]; }
+        }
+      }
+    } else {
+      for (int j = 0; j < n; ++j) {
+        double yy = w[j], s = 0.0, s2 = 0.0;
+        w[j] = 0.0;
+        for (int i = 0; i < n; ++i) s += w[i];
+        double mu = s / (double)(n - 1);
+        for (int i = 0; i < n; ++i) w2[i] = w[i] + (-mu);
+        w2[j] = 0.0;
+        for (int i = 0; i < n; ++i) s2 += w2[i] * w2[i];
+        double z = (yy - mu) / sqrt(s2 / (double)n);
+        if (fabs(z) > thresh) {
+          flag[j] = 1;
+          if (vno[j] < 0) { vno[j] = e; sc[j] = z; }
+        }
+        w[j] = yy;
+      }
+    }
+  }
+  int nb = 0;
+  for (int j = 0; j < n; ++j)
+    if (flag[j]) { bad[nb] = j; vecno[nb] = vno[j]; score[nb] = sc[j]; ++nb; }
+  free(flag); free(vno); free(sc); free(w); free(w2);
+  return nb;
+}
+
+static int form_of(const char *x) {
+  if (!x || !strcasecmp(x, "fp64")) return EIG_FORM_FP64;
+  if (!strcasecmp(x, "bf16x4")) return EIG_FORM_BF16X4;
+  if (!strcasecmp(x, "bf16x3")) return EIG_FORM_BF16X3;
+  die("unknown xtxform: %s (fp64 | bf16x4 | bf16x3)", x);
+  return -1;
+}
+
+int main(int argc, char **argv) {
+  char *parname = NULL;
+  int opt, verbose = 0;
+  while ((opt = getopt(argc, argv, "p:vV")) != -1) {
+    if (opt == 'p') parname = optarg;
+    else if (opt == 'v') printf("version: eig-mi355x smartpca 8.0.0-compatible\n");
+    else if (opt == 'V') verbose = 1;
+    else die("bad params");
+  }
+  if (!parname) {
+    fprintf(stderr, "no parameters\n");
+    return 0;
+  }
+  double t_start = now_s();
+  Params pp;
+  par_read(parname, &pp);
+  Settings S;
+  defaults(&S);
+  read_settings(&pp, &S);
+  S.verbose = verbose;
+  printf("parameter file: %s\n### THE INPUT PARAMETERS\n##PARAMETER NAME: VALUE\n", parname);
+  for (int i = 0; i < pp.n; ++i) printf("%s %s\n", pp.key[i], pp.val[i]);
+  if (!S.genotypename || !S.snpname || !S.indivname)
+    die("genotypename, snpname and indivname are required");
+  if (S.xchrom == S.numchrom + 1) S.noxdata = 0;
+  const int form = form_of(S.xtxform);
+  if (S.deletesnpoutname) remove(S.deletesnpoutname);
+
+  /* ---- inputs */
+  Snp *snps;
+  Ind *inds;
+  int is_map = 0;
+  int64_t M = read_snps(S.snpname, &snps, S.numchrom, &is_map);
+  int32_t N = read_inds(S.indivname, &inds, S.familynames);
+  int64_t rlen;
+  uint8_t *rows = read_genotypes(S.genotypename, snps, M, inds, N, S.hashcheck,
+                                 S.pordercheck && !is_map, &rlen);
+  printf("numsnps input: %lld  numindivs: %d\n", (long long)M, N);
+  /* male X heterozygotes read as missing (checkxval, mcio.c:1606-1618) */
+  if (!S.malexhet)
+    for (int64_t m = 0; m < M; ++m) {
+      if (snps[m].chrom != S.numchrom + 1) continue;
+      uint8_t *row = rows + m * rlen;
+      for (int32_t k = 0; k < N; ++k)
+        if (inds[k].sex == 'M' && geno_code(row, k) == 1) {
+          int sh = (3 - (k & 3)) << 1;
+          row[k >> 2] |= (uint8_t)(3 << sh);
+        }
+    }
+
+  /* ---- axis populations (setstatus / seteglist, mcio.c:1297-1331, egsubs.c:86-109) */
+  int neg = 0;
+  char **eglist = malloc(sizeof(char *) * (N + 1));
+  if (S.poplistname) {
+    FILE *f = fopen(S.poplistname, "r");
+    if (!f) die("can't open %s", S.poplistname);
+    char line[1024];
+    while (fgets(line, sizeof line, f)) {
+      char *p = line;
+      while (*p && isspace((unsigned char)*p)) ++p;
+      if (!*p || *p == '#') continue;
+      char *e = p;
+      while (*e && !isspace((unsigned char)*e)) ++e;
+      *e = 0;
+      eglist[neg++] = strdup(p);
+    }
+    fclose(f);
+    for (int32_t i = 0; i < N; ++i) {
+      inds[i].aff = 0;
+      if (inds[i].ignore) continue;
+      for (int k = 0; k < neg; ++k)
+        if (!strcmp(inds[i].pop, eglist[k])) inds[i].aff = 1;
+    }
+  } else {
+    for (int32_t i = 0; i < N; ++i) {
+      inds[i].aff = !inds[i].ignore;
+      if (inds[i].ignore) continue;
+      int k = 0;
+      for (; k < neg; ++k)
+        if (!strcmp(inds[i].pop, eglist[k])) break;
+      if (k == neg) eglist[neg++] = inds[i].pop;
+    }
+  }
+
+  /* ---- SNP filters (smartpca.c:801-871) */
+  for (int64_t m = 0; m < M; ++m) {
+    Snp *c = &snps[m];
+    if (c->ignore) continue;
+    const char *why = NULL;
+    if (S.noxdata && c->chrom == S.numchrom + 1) why = "chrom-X";
+    else if (c->chrom == 0) why = "chrom-0";
+    else if (c->chrom > S.numchrom + 1) why = "chrom-big";
+    else if (c->chrom == S.zchrom) why = "chrom-deltet";
+    else if (S.xchrom > 0 && c->chrom != S.xchrom) why = "not-chrom";
+    else if (S.xchrom > 0 && lround(c->ppos) < S.lopos) why = "lopos";
+    else if (S.xchrom > 0 && lround(c->ppos) > S.hipos) why = "hipos";
+    if (why) {
+      c->ignore = 1;
+      log_deleted(&S, c->id, why);
+    }
+  }
+  /* nodata: <= 1 valid genotype among non-ignored axis individuals */
+  int64_t nodata = 0;
+  for (int64_t m = 0; m < M; ++m) {
+    if (snps[m].ignore) continue;
+    const uint8_t *row = rows + m * rlen;
+    int nv = 0;
+    for (int32_t k = 0; k < N && nv <= 1; ++k)
+      if (!inds[k].ignore && inds[k].aff && geno_code(row, k) != 3) ++nv;
+    if (nv <= 1) {
+      snps[m].ignore = 1;
+      ++nodata;
+      log_deleted(&S, snps[m].id, "nodata");
+    }
+  }
+  printf("snps deleted (nodata): %lld\n", (long long)nodata);
+  /* individuals with no valid genotype over the kept SNPs (smartpca.c:885-897) */
+  {
+    int64_t *vv = calloc(N, sizeof(int64_t));
+    for (int64_t m = 0; m < M; ++m) {
+      if (snps[m].ignore) continue;
+      const uint8_t *row = rows + m * rlen;
+      for (int32_t k = 0; k < N; ++k) vv[k] += geno_code(row, k) != 3;
+    }
+    for (int32_t i = 0; i < N; ++i) {
+      if (vv[i] == 0) inds[i].ignore = 1;
+      if (!S.doproject) {
+        int k = 0;
+        for (; k < neg; ++k)
+          if (!strcmp(inds[i].pop, eglist[k])) break;
+        if (k == neg) inds[i].ignore = 1;
+      }
+    }
+    free(vv);
+  }
+  /* rmsnps: keep only the SNPs that survived */
+  int64_t Mk = 0;
+  for (int64_t m = 0; m < M; ++m)
+    if (!snps[m].ignore) {
+      if (Mk != m) {
+        memmove(rows + Mk * rlen, rows + m * rlen, (size_t)rlen);
+        snps[Mk] = snps[m];
+      }
+      ++Mk;
+    }
+  if (Mk == 0) die("no data");
+
+  /* ---- engine */
+  eig_ctx *ctx = NULL;
+  if (eig_open(S.gpu, &ctx) != EIG_OK) die("eig_open(%d) failed: no usable gfx950 device", S.gpu);
+  CHK(eig_load_packed(ctx, rows, rlen, Mk, N));
+
+  int32_t *xindex = malloc(sizeof(int32_t) * N);
+  uint8_t *dead = calloc((size_t)Mk, 1), *live = malloc((size_t)Mk);
+  int32_t *c0 = malloc(4 * Mk), *c1 = malloc(4 * Mk), *nmiss = malloc(4 * Mk),
+          *status = malloc(4 * Mk);
+  double *mean = malloc(8 * Mk), *fancy = malloc(8 * Mk);
+  double *evals = NULL, *evecs = NULL;
+  int32_t nrows = 0;
+  int64_t ncols = 0, nused = 0;
+  FILE *outl = stdout;
+  if (S.outliername) {
+    outl = fopen(S.outliername, "w");
+    if (!outl) die("can't open %s", S.outliername);
+  }
+  int numoutiter = S.numoutliter >= 1 ? S.numoutliter + 1 : 1;
+  int omode = S.numoutliter >= 1 ? S.outliermode : 2;
+  int nvec_pass = S.numeigs > S.numoutleigs ? S.numeigs : S.numoutleigs;
+  if (nvec_pass > 64) nvec_pass = 64;
+  double t_xtx = 0, t_eig = 0;
+  for (int outliter = 1; outliter <= numoutiter; ++outliter) {
+    nrows = 0;
+    for (int32_t i = 0; i < N; ++i)
+      if (!inds[i].ignore && inds[i].aff) xindex[nrows++] = i;
+    if (nrows < 2) die("too few samples: %d", nrows);
+    if (nrows >= 46341) die("size of GRM matrix too large: nrows: %d\n Run with fastmode: YES perhaps", nrows);
+    ncols = 0;
+    for (int64_t m = 0; m < Mk; ++m) {
+      live[m] = !dead[m];
+      ncols += live[m];
+    }
+    double t0 = now_s();
+    CHK(eig_select(ctx, xindex, nrows));
+    CHK(eig_snpstats(ctx, live, S.altnormstyle, S.fancynorm, S.minallelecnt, S.maxmissing, c0,
+                     c1, nmiss, mean, fancy, status));
+    int nkill = 0;
+    nused = 0;
+    for (int64_t m = 0; m < Mk; ++m) {
+      if (status[m] == EIG_SNP_DEAD) continue;
+      if (status[m] != EIG_SNP_KEPT) {
+        dead[m] = 1;
+        log_deleted(&S, snps[m].id, "minallelecnt");
+        if (nkill < 10) {
+          int tt = status[m] == EIG_SNP_ALLMISS ? 0 : (c0[m] < c1[m] ? c0[m] : c1[m]);
+          printf(" snp %20s ignored . allelecnt: %5d  missing: %5d\n", snps[m].id, tt,
+                 status[m] == EIG_SNP_ALLMISS ? 0 : nmiss[m]);
+        }
+        ++nkill;
+      } else {
+        ++nused;
+      }
+    }
+    printf("number of samples used: %d number of snps used: %lld\n", nrows, (long long)nused);
+    double yscale = 0.0;
+    int rc = eig_xtx(ctx, form, S.bf16block, &yscale, NULL);
+    if (rc == EIG_E_NUMERIC) die("%s", eig_last_error(ctx));
+    CHK(rc);
+    double t1 = now_s();
+    free(evals);
+    free(evecs);
+    evals = malloc(sizeof(double) * nrows);
+    int nv = nvec_pass < nrows ? nvec_pass : nrows;
+    evecs = malloc(sizeof(double) * (size_t)nrows * (nv > 0 ? nv : 1));
+    CHK(eig_eigh_xtx(ctx, nv, evals, evecs));
+    double t2 = now_s();
+    t_xtx += t1 - t0;
+    t_eig += t2 - t1;
+    if (outliter > S.numoutliter) break;
+    int nle = S.numoutleigs < nrows - 1 ? S.numoutleigs : nrows - 1;
+    if (nle > nv) nle = nv;
+    int *bad = malloc(sizeof(int) * nrows), *vno = malloc(sizeof(int) * nrows);
+    double *sc = malloc(sizeof(double) * nrows);
+    int nbad = ridoutlier(evecs, nrows, nle, S.outlthresh, omode, bad, vno, sc);
+    for (int b = 0; b < nbad; ++b) {
+      Ind *x = &inds[xindex[bad[b]]];
+      fprintf(outl, "REMOVED outlier %s iter %d evec %d sigmage %.3f pop: %s\n", x->id, outliter,
+              vno[b], sc[b], x->pop);
+      x->ignore = 1;
+    }
+    free(bad);
+    free(vno);
+    free(sc);
+    if (nbad == 0) break;
+    int nr = 0;
+    for (int32_t i = 0; i < N; ++i) nr += !inds[i].ignore && inds[i].aff;
+    printf("number of samples after outlier removal: %d\n", nr);
+  }
+  if (outl != stdout) fclose(outl);
+
+  /* ---- GRM (dumpgrm, smartpca.c:3687-3808) */
+  if (S.grmoutname) {
+    double *xtx = malloc(sizeof(double) * (size_t)nrows * nrows);
+    double ys;
+    CHK(eig_xtx(ctx, form, S.bf16block, &ys, xtx));
+    char nm[4096];
+    snprintf(nm, sizeof nm, "%s.id", S.grmoutname);
+    FILE *f = fopen(nm, "w");
+    if (!f) die("can't open %s", nm);
+    for (int32_t a = 0; a < nrows; ++a) fprintf(f, "%s\t%s\n", "NA", inds[xindex[a]].id);
+    fclose(f);
+    double tr = 0.0;
+    for (int32_t a = 0; a < nrows; ++a) tr += xtx[(int64_t)a * nrows + a];
+    if (S.grmbinary) {
+      snprintf(nm, sizeof nm, "%s.N.bin", S.grmoutname);
+      f = fopen(nm, "wb");
+      if (!f) die("can't open %s", nm);
+      int32_t w = (int32_t)nused;
+      for (int64_t a = 0; a < (int64_t)nrows * (nrows + 1) / 2; ++a) fwrite(&w, 4, 1, f);
+      fclose(f);
+      snprintf(nm, sizeof nm, "%s.bin", S.grmoutname);
+      f = fopen(nm, "wb");
+      if (!f) die("can't open %s", nm);
+      double yn = tr / (double)nrows;
+      for (int32_t a = 0; a < nrows; ++a)
+        for (int32_t b = 0; b <= a; ++b) {
+          float v = (float)(xtx[(int64_t)a * nrows + b] / yn);
+          fwrite(&v, 4, 1, f);
+        }
+      fclose(f);
+    } else {
+      f = fopen(S.grmoutname, "w");
+      if (!f) die("can't open %s", S.grmoutname);
+      double rec = (double)nrows / tr;
+      for (int32_t a = 0; a < nrows; ++a)
+        for (int32_t b = 0; b <= a; ++b)
+          fprintf(f, "%d %d %lld %0.6f\n", a + 1, b + 1, (long long)nused,
+                  xtx[(int64_t)a * nrows + b] * rec);
+      fclose(f);
+    }
+    free(xtx);
+    printf("grm dumped\n");
+  }
+
+  /* ---- outputs (smartpca.c:1370-1591) */
+  int K = S.numeigs;
+  if (K > nrows) K = nrows;
+  if (K > ncols) K = (int)ncols;
+  int nvec_got = nvec_pass < nrows ? nvec_pass : nrows;
+  if (K > nvec_got) K = nvec_got;
+  int npos = 0;
+  for (int32_t i = 0; i < nrows; ++i) npos += evals[i] > 0.0;
+  if (npos == 0) die("no data");
+  if (S.outputvname) {
+    FILE *f = fopen(S.outputvname, "w");
+    if (!f) die("can't open %s", S.outputvname);
+    for (int32_t i = 0; i < nrows; ++i) fprintf(f, "%12.6f\n", evals[i]);
+    fclose(f);
+  }
+  FILE *ofile = stdout;
+  if (S.outputname) {
+    ofile = fopen(S.outputname, "w");
+    if (!ofile) die("can't open %s", S.outputname);
+  }
+  if (K > 0) {
+    fprintf(ofile, "%20s ", "#eigvals:");
+    for (int j = 0; j < K; ++j) fprintf(ofile, "%9.3f ", evals[j]);
+    fprintf(ofile, "\n");
+    /* loadings + projection scales over the SNPs that survived the final pass */
+    uint8_t *ign = malloc((size_t)Mk);
+    for (int64_t m = 0; m < Mk; ++m) ign[m] = dead[m] || status[m] != EIG_SNP_KEPT;
+    double *ffvecs = malloc(sizeof(double) * (size_t)K * Mk), fxscal[64];
+    CHK(eig_loadings(ctx, evecs, K, ign, ffvecs, fxscal));
+    double *acoeffs = calloc((size_t)K * N, sizeof(double));
+    double *bcoeffs = calloc((size_t)K * N, sizeof(double));
+    if (S.regmode) {
+      int32_t *use = malloc(4 * (size_t)N), *lst = malloc(4 * (size_t)N);
+      for (int32_t i = 0; i < N; ++i) use[i] = !inds[i].ignore;
+      CHK(eig_lsqproject(ctx, use, acoeffs, bcoeffs, lst));
+      for (int32_t i = 0; i < N; ++i)
+        if (lst[i] == 1) {
+          inds[i].ignore = 1;
+          printf("%s ignored (insufficient data\n", inds[i].id);
+        }
+      /* seteigscale (smartpca.c:4758-4782) over the axis samples */
+      for (int j = 0; j < K; ++j) {
+        double ab = 0.0, aa = 0.0;
+        for (int32_t r = 0; r < nrows; ++r) {
+          double a = acoeffs[(int64_t)j * N + xindex[r]], b = bcoeffs[(int64_t)j * N + xindex[r]];
+          ab += a * b;
+          aa += a * a;
+        }
+        double es = ab / aa;
+        for (int32_t i = 0; i < N; ++i) acoeffs[(int64_t)j * N + i] *= es;
+      }
+      free(use);
+      free(lst);
+    } else {
+      printf("lsqproject off:  deprecated\n");
+    }
+    for (int32_t i = 0; i < N; ++i) {
+      if (inds[i].ignore) continue;
+      fprintf(ofile, "%20s ", inds[i].id);
+      for (int j = 0; j < K; ++j)
+        fprintf(ofile, S.hiprec ? "%12.6f  " : "%10.4f  ", acoeffs[(int64_t)j * N + i]);
+      fprintf(ofile, "%15s\n", inds[i].pop);
+    }
+    /* snp weights (writesnpeigs, smartpca.c:3038-3104) */
+    if (S.snpeigname) {
+      FILE *f = fopen(S.snpeigname, "w");
+      if (!f) die("can't open %s", S.snpeigname);
+      for (int j = 0; j < K; ++j) {
+        double y = 0.0;
+        for (int64_t m = 0; m < Mk; ++m) y += ffvecs[(int64_t)j * Mk + m] * ffvecs[(int64_t)j * Mk + m];
+        double sc = sqrt((double)ncols / y);
+        for (int64_t m = 0; m < Mk; ++m) ffvecs[(int64_t)j * Mk + m] *= sc;
+      }
+      for (int64_t m = 0; m < Mk; ++m) {
+        if (ign[m]) continue;
+        fprintf(f, "%20s %2d %12ld", snps[m].id, snps[m].chrom, lround(snps[m].ppos));
+        for (int j = 0; j < K; ++j)
+          fprintf(f, S.hiprec ? " %12.6f" : " %9.3f", ffvecs[(int64_t)j * Mk + m]);
+        fprintf(f, "\n");
+      }
+      fclose(f);
+    }
+    free(ign);
+    free(ffvecs);
+    free(acoeffs);
+    free(bcoeffs);
+  }
+  if (ofile != stdout) fclose(ofile);
+  printf("##end of smartpca: %12.3f seconds wall (xtx %.3f, eigensolve %.3f)\n",
+         now_s() - t_start, t_xtx, t_eig);
+  eig_close(ctx);
+  return 0;
+}

@@ -1,0 +1,197 @@
+"""The smartpca drop-in end to end: same parfile, same inputs, same .evec/.eval.
+
+GPU tests run the CLI (eig_amd/bin/smartpca, C host + libeig_amd.so) and compare
+its files with the reference's own example outputs (byte for byte) and with the
+CPU oracle's pipeline on seeded synthetic panels (numeric tolerance stated per
+test).  CPU tests cover the fault paths that end before any device call and the
+exported C-ABI.
+"""
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "eig_amd", "bin", "smartpca")
+HDR = os.path.join(ROOT, "include", "eig_abi.h")
+
+
+def run_cli(par_path, cwd, expect_ok=True):
+    r = subprocess.run([CLI, "-p", par_path], cwd=cwd, stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=300)
+    print(r.stdout[-3000:], r.stderr[-2000:], flush=True)
+    if expect_ok:
+        assert r.returncode == 0, r.stderr
+    return r
+
+
+def write_par(path, **kv):
+    with open(path, "w") as f:
+        for k, v in kv.items():
+            f.write(f"{k}: {v}\n")
+
+
+def _copy_example(golden, tmp_path):
+    for f in ("example.geno", "example.snp", "example.ind"):
+        shutil.copy(os.path.join(golden, "eigenstrat", f), tmp_path / f)
+    d = os.path.join(golden, "convertf")
+    shutil.copy(os.path.join(d, "example.packedancestrymapgeno"), tmp_path / "packed.geno")
+    shutil.copy(os.path.join(d, "example.snp"), tmp_path / "packed.snp")
+    shutil.copy(os.path.join(d, "example.ind"), tmp_path / "packed.ind")
+    shutil.copy(os.path.join(d, "example.bed"), tmp_path / "example.bed")
+    shutil.copy(os.path.join(d, "example.pedsnp"), tmp_path / "example.bim")
+    shutil.copy(os.path.join(d, "example.pedind"), tmp_path / "example.fam")
+
+
+# ------------------------------------------------------------------ CPU tests
+def test_cli_and_library_are_built():
+    assert os.path.exists(CLI), "run python -m eig_amd.build"
+    r = subprocess.run([CLI], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    assert r.returncode == 0 and "no parameters" in r.stdout + r.stderr
+
+
+def test_library_exports_every_declared_symbol():
+    import ctypes
+    import eig_amd
+    lib = ctypes.CDLL(eig_amd.LIB_PATH)
+    decl = re.findall(r"^\w[\w\s\*]*?\b(eig_\w+)\s*\(", open(HDR).read(), re.M)
+    assert len(decl) >= 14
+    for name in decl:
+        assert hasattr(lib, name), name
+    assert set(decl) <= set(eig_amd.SIGNATURES), set(decl) - set(eig_amd.SIGNATURES)
+
+
+def test_duplicate_parameter_is_fatal(tmp_path):
+    p = tmp_path / "par"
+    p.write_text("genotypename: a\ngenotypename: b\n")
+    r = run_cli(str(p), tmp_path, expect_ok=False)
+    assert r.returncode != 0 and "duplicate parameter" in r.stderr
+
+
+def test_geno_hash_mismatch_is_fatal(golden, tmp_path):
+    _copy_example(golden, tmp_path)
+    lines = (tmp_path / "packed.ind").read_text().splitlines()
+    lines[0] = lines[0].replace("SAMPLE0", "SAMPLEX")
+    (tmp_path / "packed.ind").write_text("\n".join(lines) + "\n")
+    write_par(tmp_path / "par", genotypename="packed.geno", snpname="packed.snp",
+              indivname="packed.ind")
+    r = run_cli(str(tmp_path / "par"), tmp_path, expect_ok=False)
+    assert r.returncode != 0 and "indiv file has changed" in r.stderr
+
+
+def test_truncated_geno_is_fatal(golden, tmp_path):
+    _copy_example(golden, tmp_path)
+    raw = (tmp_path / "packed.geno").read_bytes()
+    (tmp_path / "packed.geno").write_bytes(raw[:-20])
+    write_par(tmp_path / "par", genotypename="packed.geno", snpname="packed.snp",
+              indivname="packed.ind")
+    r = run_cli(str(tmp_path / "par"), tmp_path, expect_ok=False)
+    assert r.returncode != 0 and "length mismatch" in r.stderr
+
+
+def test_unordered_snps_with_packed_input_is_fatal(golden, tmp_path):
+    _copy_example(golden, tmp_path)
+    lines = (tmp_path / "packed.snp").read_text().splitlines()
+    lines[1], lines[2] = lines[2], lines[1]
+    (tmp_path / "packed.snp").write_text("\n".join(lines) + "\n")
+    write_par(tmp_path / "par", genotypename="packed.geno", snpname="packed.snp",
+              indivname="packed.ind", hashcheck="NO")
+    r = run_cli(str(tmp_path / "par"), tmp_path, expect_ok=False)
+    assert r.returncode != 0 and "out of order" in r.stderr
+
+
+# ------------------------------------------------------------------ GPU tests
+@pytest.mark.gpu
+@pytest.mark.parametrize("fmt", ["eigenstrat", "packed", "bed"])
+def test_cli_reproduces_reference_example(golden, tmp_path, fmt):
+    _copy_example(golden, tmp_path)
+    inputs = {"eigenstrat": ("example.geno", "example.snp", "example.ind"),
+              "packed": ("packed.geno", "packed.snp", "packed.ind"),
+              "bed": ("example.bed", "example.bim", "example.fam")}[fmt]
+    write_par(tmp_path / "par", genotypename=inputs[0], snpname=inputs[1], indivname=inputs[2],
+              evecoutname="out.evec", evaloutname="out.eval", altnormstyle="NO",
+              numoutevec=2, numoutlieriter=5, numoutlierevec=2, outliersigmathresh=6.0,
+              familynames="NO", grmoutname="grm")
+    print(f"[gpu-case] cli example {fmt}", flush=True)
+    run_cli(str(tmp_path / "par"), tmp_path)
+    d = os.path.join(golden, "eigenstrat")
+    assert (tmp_path / "out.eval").read_text() == open(os.path.join(d, "example.eval")).read()
+    assert (tmp_path / "out.evec").read_text() == open(os.path.join(d, "example.pca.evec")).read()
+    got = [l.split() for l in (tmp_path / "grm").read_text().splitlines()]
+    want = [l.split() for l in open(os.path.join(golden, "popgen", "grmjunk"))]
+    assert [g[:3] for g in got] == [w[:3] for w in want]
+    assert np.allclose([float(g[3]) for g in got], [float(w[3]) for w in want], atol=1.1e-6)
+    assert (tmp_path / "grm.id").read_text() == open(os.path.join(golden, "popgen", "grmjunk.id")).read()
+
+
+def _synthetic(tmp_path, nsnp, scale, cfg=1, plant=None, miss_rows=None):
+    from eig_amd import synth
+    P = synth.config_panel(cfg, nsnp=nsnp, scale=scale)
+    prefix = str(tmp_path / "syn")
+    synth.write_files(P, prefix)
+    if plant is not None:
+        # replace individual `plant` by an extreme genotype vector (all hom-alt)
+        raw = bytearray(open(prefix + ".geno", "rb").read())
+        rlen = max(48, (2 * P.nind + 7) // 8)
+        for m in range(P.nsnp):
+            off = rlen * (m + 1) + plant // 4
+            sh = 2 * (3 - plant % 4)
+            raw[off] = (raw[off] & ~(3 << sh)) | (2 << sh)
+        open(prefix + ".geno", "wb").write(bytes(raw))
+    return P, prefix
+
+
+def _parse_evec(text):
+    lines = text.splitlines()
+    head = [float(x) for x in lines[0].split()[1:]]
+    rows = [(l.split()[0], [float(x) for x in l.split()[1:-1]], l.split()[-1]) for l in lines[1:]]
+    return head, rows
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", ["fp64", "bf16x4"])
+def test_cli_matches_oracle_on_synthetic_panel(tmp_path, form):
+    P, prefix = _synthetic(tmp_path, 6000, 0.3)
+    par = dict(genotypename=prefix + ".geno", snpname=prefix + ".snp",
+               indivname=prefix + ".ind", evecoutname="out.evec", evaloutname="out.eval",
+               numoutevec=10, numoutlieriter=0, xtxform=form)
+    write_par(tmp_path / "par", **par)
+    print(f"[gpu-case] cli synthetic N={P.nind} M={P.nsnp} form={form}", flush=True)
+    run_cli(str(tmp_path / "par"), tmp_path)
+    ref = O.smartpca({k: str(v) for k, v in par.items()}, str(tmp_path))
+    got_eval = np.array([float(x) for x in (tmp_path / "out.eval").read_text().split()])
+    want_eval = np.array([float(x) for x in ref["eval_lines"]])
+    assert np.max(np.abs(got_eval - want_eval)) <= 2e-6
+    gh, grows = _parse_evec((tmp_path / "out.evec").read_text())
+    wh, wrows = _parse_evec("\n".join(ref["evec_lines"]))
+    assert np.allclose(gh, wh, atol=1.01e-3)
+    assert [r[0] for r in grows] == [r[0] for r in wrows]
+    assert [r[2] for r in grows] == [r[2] for r in wrows]
+    diff = np.max(np.abs(np.array([r[1] for r in grows]) - np.array([r[1] for r in wrows])))
+    assert diff <= 2.01e-4, diff  # %10.4f printing: at most a last-digit flip
+
+
+@pytest.mark.gpu
+def test_cli_outlier_removal_matches_oracle(tmp_path):
+    P, prefix = _synthetic(tmp_path, 3000, 0.15, plant=7)
+    par = dict(genotypename=prefix + ".geno", snpname=prefix + ".snp",
+               indivname=prefix + ".ind", evecoutname="out.evec", evaloutname="out.eval",
+               outlieroutname="out.outl", numoutevec=4, numoutlieriter=3, numoutlierevec=4,
+               outliersigmathresh=6.0)
+    write_par(tmp_path / "par", **par)
+    print(f"[gpu-case] cli outliers N={P.nind} M={P.nsnp}", flush=True)
+    run_cli(str(tmp_path / "par"), tmp_path)
+    ref = O.smartpca({k: str(v) for k, v in par.items()}, str(tmp_path))
+    got = (tmp_path / "out.outl").read_text().splitlines()
+    want = ["REMOVED outlier %s iter %d evec %d sigmage %.3f pop: %s" % r for r in ref["removed"]]
+    assert len(want) >= 1
+    assert [g.split()[:7] for g in got] == [w.split()[:7] for w in want]
+    assert np.allclose([float(g.split()[8]) for g in got], [float(w.split()[8]) for w in want],
+                       atol=2e-3)
+    ids = [l.split()[0] for l in (tmp_path / "out.evec").read_text().splitlines()[1:]]
+    assert P.ids()[0][7] not in ids
