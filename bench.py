@@ -72,6 +72,98 @@ def cpu_baseline(eng, panel, nthreads):
                        f"LAPACK dspev at n={ne} ({t_e:.2f}s, x(N/n)^3); per pass")
 
 
+def serve_leg(eng, P, vec, seconds, rate=1000.0, pool_n=4096):
+    """lsqproject serving against the resident panel: Poisson arrivals at `rate`
+    samples/s, request sizes log-uniform on [1, 512], query samples from the panel's
+    populations with per-sample missingness ~ U(0, 0.9), rows resident in HBM.
+    Returns latency percentiles (submit -> result ready), samples/s, and the same
+    for a host-memory (PCIe-inclusive) run and a saturating burst."""
+    import threading
+
+    import eig_amd
+    from eig_amd import synth
+    K = vec.shape[0]
+    N = P.nind
+    eng.loadings(vec, want_ffvecs=False)
+    a, b, st = eng.lsqproject(np.ones(N, np.int32), K)
+    eigscale = np.array([np.dot(a[j], b[j]) / np.dot(a[j], a[j]) for j in range(K)])
+    stride = ((P.nsnp + 63) // 64) * 16
+    pop, mt = synth.serving_queries(P, pool_n, seed=0xE16 + 5)
+    pool = eng.dev_alloc(pool_n * stride)
+    eig_amd.synth_queries(eng, P.thr, pop, mt, 0xE16 + 5, pool, stride)
+    host_ptr, host = eng.host_alloc(pool_n * stride)  # pinned
+    eng.memcpy(host_ptr, pool, pool_n * stride)
+    host_np = host.reshape(pool_n, stride)
+    srv = eig_amd.Server(eng, eigscale=eigscale, nlanes=4, max_batch=1024)
+    info0 = srv.info()
+    log(f"server up: chunk {info0['chunk']} grid {info0['grid']}")
+    rng = np.random.default_rng(0xE16 + 5)
+
+    def schedule(dur):
+        t, out = 0.0, []
+        while True:
+            n = int(min(512, max(1, np.floor(np.exp(rng.uniform(0, np.log(513)))))))
+            t += rng.exponential(1.0 / (rate / 81.9))
+            if t > dur:
+                return out
+            out.append((t, n, int(rng.integers(0, pool_n - n))))
+
+    def run(reqs, on_device):
+        lat, done = [], []
+        q = []
+        cv = threading.Condition()
+
+        def poller():
+            k = 0
+            while k < len(reqs):
+                with cv:
+                    while len(q) <= k:
+                        cv.wait()
+                    t = q[k]
+                co, stt, ms = srv.poll(t)
+                lat.append(ms)
+                done.append(time.perf_counter())
+                k += 1
+        th = threading.Thread(target=poller)
+        th.start()
+        t0 = time.perf_counter()
+        for (ta, n, off) in reqs:
+            dt = t0 + ta - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            if on_device:
+                t = srv.submit(device_ptr=pool + off * stride, nq=n, stride=stride)
+            else:
+                t = srv.submit(host_np[off:off + n])
+            with cv:
+                q.append(t)
+                cv.notify()
+        th.join()
+        wall = max(done) - t0
+        ns = sum(n for _, n, _ in reqs)
+        lat = np.array(lat)
+        return dict(p50_ms=round(float(np.percentile(lat, 50)), 3),
+                    p99_ms=round(float(np.percentile(lat, 99)), 3),
+                    max_ms=round(float(lat.max()), 3), requests=len(reqs), samples=ns,
+                    samples_per_s=round(ns / wall, 1))
+    run(schedule(1.0), True)  # warm-up
+    b0 = srv.info()["batches"]
+    res = run(schedule(seconds), True)
+    res["batches"] = srv.info()["batches"] - b0
+    log(f"serving (HBM-resident rows): {res}")
+    pcie = run(schedule(seconds / 2), False)
+    log(f"serving (host rows, PCIe-inclusive): {pcie}")
+    burst = [(0.0, n, off) for (_, n, off) in schedule(seconds / 2)]
+    sat = run(burst, True)
+    log(f"serving saturation burst: {sat}")
+    srv.stop()
+    eng.host_free(host_ptr)
+    eng.dev_free(pool)
+    return dict(resident=res, host_rows=pcie, burst=dict(samples=sat["samples"],
+                samples_per_s=sat["samples_per_s"], p99_ms=sat["p99_ms"]),
+                chunk=info0["chunk"], grid=info0["grid"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +173,7 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported",
@@ -154,6 +247,13 @@ def main():
         "fp64_syrk_roofline": {"achieved": round(N * (N + 1) * M / (f["syrk_ms"] * 1e-3) / 1e12, 2),
                                "peak": PEAK_FP64, "unit": "TFLOP/s"},
     }
+    if not args.no_serve:
+        log("serving leg")
+        sv = serve_leg(eng, P, f["vec"], args.serve_seconds)
+        out["lsqproject_p99_ms"] = sv["resident"]["p99_ms"]
+        out["lsqproject_p50_ms"] = sv["resident"]["p50_ms"]
+        out["lsqproject_samples_per_s"] = sv["resident"]["samples_per_s"]
+        out["serving"] = sv
     if not args.no_cpu:
         nthreads = os.cpu_count() or 1
         nthreads = nthreads if nthreads <= 8 else min(nthreads - 1, 127)

@@ -45,7 +45,26 @@ SIGNATURES = {
     "eig_eigh": (C.c_int, [_vp, _f64p, C.c_int32, C.c_int32, _f64p, _vp]),
     "eig_loadings": (C.c_int, [_vp, _f64p, C.c_int32, _vp, _vp, _vp]),
     "eig_lsqproject": (C.c_int, [_vp, _i32p, _f64p, _f64p, _i32p]),
+    "eig_dev_alloc": (C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    "eig_dev_free": (C.c_int, [_vp, _vp]),
+    "eig_host_alloc": (C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    "eig_host_free": (C.c_int, [_vp, _vp]),
+    "eig_memcpy": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "eig_server_start": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.POINTER(_vp)]),
+    "eig_server_submit": (C.c_int, [_vp, _vp, C.c_int64, C.c_int32, C.c_int32,
+                                    C.POINTER(C.c_int64)]),
+    "eig_server_poll": (C.c_int, [_vp, C.c_int64, C.c_int32, _vp, _vp, C.POINTER(C.c_double)]),
+    "eig_server_info": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_int64)]),
+    "eig_server_error": (C.c_char_p, [_vp]),
+    "eig_server_stop": (None, [_vp]),
+    "eig_synth_queries": (C.c_int, [_vp, C.c_int64, C.c_int32, C.c_int32, _u32p, _i32p, _u32p,
+                                    C.c_uint64, _vp, C.c_int64]),
 }
+
+EIG_PENDING = 1
 
 
 def lib():
@@ -103,6 +122,31 @@ class Engine:
 
     def last_ms(self, what: str) -> float:
         return self._L.eig_last_ms(self.h, what.encode())
+
+    # ---- device / pinned memory (this package loads no other HIP runtime)
+    def dev_alloc(self, nbytes: int) -> int:
+        p = _vp()
+        self._chk(self._L.eig_dev_alloc(self.h, nbytes, C.byref(p)), "eig_dev_alloc")
+        return p.value
+
+    def dev_free(self, ptr: int):
+        self._chk(self._L.eig_dev_free(self.h, C.c_void_p(ptr)), "eig_dev_free")
+
+    def host_alloc(self, nbytes: int):
+        """Pinned host buffer as (address, uint8 numpy view)."""
+        p = _vp()
+        self._chk(self._L.eig_host_alloc(self.h, nbytes, C.byref(p)), "eig_host_alloc")
+        view = np.ctypeslib.as_array((C.c_uint8 * nbytes).from_address(p.value))
+        return p.value, view
+
+    def host_free(self, ptr: int):
+        self._chk(self._L.eig_host_free(self.h, C.c_void_p(ptr)), "eig_host_free")
+
+    def memcpy(self, dst, src, nbytes: int):
+        """dst / src: integer addresses (device or host) or numpy arrays."""
+        d = dst.ctypes.data if isinstance(dst, np.ndarray) else dst
+        s_ = src.ctypes.data if isinstance(src, np.ndarray) else src
+        self._chk(self._L.eig_memcpy(self.h, C.c_void_p(d), C.c_void_p(s_), nbytes), "eig_memcpy")
 
     def load_packed(self, rows: np.ndarray, nind: int):
         rows = np.ascontiguousarray(rows, dtype=np.uint8)
@@ -180,3 +224,89 @@ class Engine:
         self._chk(self._L.eig_lsqproject(self.h, use, a.reshape(-1), b.reshape(-1), st),
                   "eig_lsqproject")
         return a, b, st
+
+
+class Server:
+    """lsqproject server (eig_server_*) started from an Engine whose loadings are
+    current.  Query rows are sample-major packed genotypes (see eig_abi.h)."""
+
+    def __init__(self, eng: Engine, eigscale=None, nlanes=0, max_batch=0, chunk=0):
+        self._L = eng._L
+        es = None if eigscale is None else np.ascontiguousarray(eigscale, dtype=np.float64)
+        h = _vp()
+        rc = self._L.eig_server_start(eng.h, _ptr(es), nlanes, max_batch, chunk, C.byref(h))
+        if rc != 0:
+            eng._chk(rc, "eig_server_start")
+        self.h = h
+        self._keep = {}
+        info = self.info()
+        self.K, self.nsnp, self.rowpad = info["K"], info["nsnp"], info["rowpad"]
+        self.rowbytes = (self.nsnp + 3) // 4
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self._L.eig_server_error(self.h)
+            raise EigError(f"{what}: rc={rc}: {msg.decode() if msg else ''}")
+
+    def info(self):
+        K = C.c_int32(); M = C.c_int64(); rp = C.c_int64(); ch = C.c_int32(); g = C.c_int32()
+        nb = C.c_int64(); ns = C.c_int64()
+        self._chk(self._L.eig_server_info(self.h, C.byref(K), C.byref(M), C.byref(rp),
+                                          C.byref(ch), C.byref(g), C.byref(nb), C.byref(ns)),
+                  "eig_server_info")
+        return dict(K=K.value, nsnp=M.value, rowpad=rp.value, chunk=ch.value, grid=g.value,
+                    batches=nb.value, samples=ns.value)
+
+    def submit(self, rows=None, nq=None, stride=None, device_ptr=None):
+        """Host rows: uint8 array [nq, >= ceil(nsnp/4)] (kept alive until polled).
+        Device rows: device_ptr (int address), stride and nq."""
+        t = C.c_int64()
+        if device_ptr is not None:
+            rc = self._L.eig_server_submit(self.h, C.c_void_p(device_ptr), stride, nq, 1,
+                                           C.byref(t))
+            self._chk(rc, "eig_server_submit")
+        else:
+            rows = np.ascontiguousarray(rows, dtype=np.uint8)
+            rc = self._L.eig_server_submit(self.h, _ptr(rows), rows.shape[1], rows.shape[0], 0,
+                                           C.byref(t))
+            self._chk(rc, "eig_server_submit")
+            self._keep[t.value] = rows
+            nq = rows.shape[0]
+        self._keep.setdefault(t.value, None)
+        self._keep[("n", t.value)] = nq
+        return t.value
+
+    def poll(self, ticket, wait=True):
+        """(coords [nq, K], status [nq], latency_ms) or None while pending."""
+        nq = self._keep[("n", ticket)]
+        co = np.zeros((nq, self.K)); st = np.zeros(nq, np.int32); lat = C.c_double()
+        rc = self._L.eig_server_poll(self.h, ticket, 1 if wait else 0, _ptr(co), _ptr(st),
+                                     C.byref(lat))
+        if rc == EIG_PENDING:
+            return None
+        self._keep.pop(ticket, None)
+        self._keep.pop(("n", ticket), None)
+        self._chk(rc, "eig_server_poll")
+        return co, st, lat.value
+
+    def stop(self):
+        if self.h:
+            self._L.eig_server_stop(self.h)
+            self.h = None
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.stop()
+
+
+def synth_queries(eng: Engine, thr, pop, miss_thr, seed, rows_dev_ptr, stride):
+    """eig_synth_queries into caller device memory (integer address)."""
+    thr = np.ascontiguousarray(thr, dtype=np.uint32)
+    pop = np.ascontiguousarray(pop, dtype=np.int32)
+    miss_thr = np.ascontiguousarray(miss_thr, dtype=np.uint32)
+    eng._chk(eng._L.eig_synth_queries(eng.h, thr.shape[1], len(pop), thr.shape[0], thr.reshape(-1),
+                                      pop, miss_thr, seed, C.c_void_p(rows_dev_ptr), stride),
+             "eig_synth_queries")

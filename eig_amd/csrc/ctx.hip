@@ -124,6 +124,49 @@ extern "C" int eig_load_packed(eig_ctx *ctx, const uint8_t *rows, int64_t rlen, 
   return EIG_OK;
 }
 
+// ---- caller memory on the context's device -----------------------------------
+extern "C" int eig_dev_alloc(eig_ctx *ctx, size_t bytes, void **ptr) {
+  if (!ctx || !ptr || bytes == 0) return eig_set_err(ctx, EIG_E_ARG, "eig_dev_alloc: args");
+  EIG_HIP(ctx, hipSetDevice(ctx->device));
+  if (hipMalloc(ptr, bytes) != hipSuccess) {
+    *ptr = nullptr;
+    return eig_set_err(ctx, EIG_E_NOMEM, "eig_dev_alloc(%zu)", bytes);
+  }
+  return EIG_OK;
+}
+
+extern "C" int eig_dev_free(eig_ctx *ctx, void *ptr) {
+  if (!ctx) return EIG_E_ARG;
+  EIG_HIP(ctx, hipSetDevice(ctx->device));
+  EIG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  EIG_HIP(ctx, hipFree(ptr));
+  return EIG_OK;
+}
+
+extern "C" int eig_host_alloc(eig_ctx *ctx, size_t bytes, void **ptr) {
+  if (!ctx || !ptr || bytes == 0) return eig_set_err(ctx, EIG_E_ARG, "eig_host_alloc: args");
+  EIG_HIP(ctx, hipSetDevice(ctx->device));
+  if (hipHostMalloc(ptr, bytes, hipHostMallocDefault) != hipSuccess) {
+    *ptr = nullptr;
+    return eig_set_err(ctx, EIG_E_NOMEM, "eig_host_alloc(%zu)", bytes);
+  }
+  return EIG_OK;
+}
+
+extern "C" int eig_host_free(eig_ctx *ctx, void *ptr) {
+  if (!ctx) return EIG_E_ARG;
+  EIG_HIP(ctx, hipHostFree(ptr));
+  return EIG_OK;
+}
+
+extern "C" int eig_memcpy(eig_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return eig_set_err(ctx, EIG_E_ARG, "eig_memcpy: args");
+  EIG_HIP(ctx, hipSetDevice(ctx->device));
+  EIG_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, ctx->stream));
+  EIG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return EIG_OK;
+}
+
 extern "C" int eig_fetch_packed(eig_ctx *ctx, int64_t m0, int64_t count, uint8_t *rows) {
   if (!ctx || !rows || m0 < 0 || count < 0 || m0 + count > ctx->nsnp)
     return eig_set_err(ctx, EIG_E_ARG, "eig_fetch_packed: range");

@@ -15,6 +15,10 @@
 constexpr int kTileS = 128;
 constexpr int kTileWords = kTileS / 16;
 
+// Largest K of loadings/lsqproject; the fxscal buffer holds K scales in its first
+// kMaxK slots and xmean = mean*fancy after them.
+constexpr int kMaxK = 16;
+
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;

@@ -16,7 +16,6 @@
 
 #include "eig_internal.h"
 
-constexpr int kMaxK = 16;
 constexpr int kLsqChunk = 4096;  // SNPs per lsqproject partial
 
 // run the statement with the compile-time constant KC equal to the runtime K (1..16)

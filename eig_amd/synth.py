@@ -111,6 +111,49 @@ class Panel:
         return inds, pops, snps
 
 
+def query_genotypes(thr, pop, miss_thr, seed, nsnp=None):
+    """CPU restatement of eig_synth_queries: int32 [nq, nsnp] codes, 3 = missing.
+    Sample q draws both allele copies from population pop[q]; key = q*nsnp + m."""
+    thr = np.asarray(thr, dtype=np.uint32)
+    nsnp = thr.shape[1] if nsnp is None else nsnp
+    pop = np.asarray(pop)
+    q = np.arange(len(pop), dtype=np.uint64)[:, None]
+    m = np.arange(nsnp, dtype=np.uint64)[None, :]
+    key = q * np.uint64(nsnp) + m
+    with np.errstate(over="ignore"):
+        h1 = _mix64(np.uint64(seed) + (key + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15))
+        h2 = _mix64(h1 ^ np.uint64(0x5851F42D4C957F2D))
+    t = thr[pop][:, :nsnp].astype(np.uint64)
+    lo = h1 & np.uint64(0xFFFFFFFF)
+    hi = h1 >> np.uint64(32)
+    g = (lo < t).astype(np.int32) + (hi < t).astype(np.int32)
+    miss = (h2 & np.uint64(0xFFFFFFFF)) < np.asarray(miss_thr, dtype=np.uint64)[:, None]
+    g[miss] = 3
+    return g
+
+
+def pack_samples(g: np.ndarray, stride=None) -> np.ndarray:
+    """Sample-major packed rows of codes g [nq, nsnp]: SNP m at byte m/4, bits
+    7-6, 5-4, 3-2, 1-0 for m%4 = 0..3; padding codes are 3."""
+    nq, M = g.shape
+    nb = (M + 3) // 4
+    stride = nb if stride is None else stride
+    pad = np.full((nq, stride * 4), 3, np.uint8)
+    pad[:, :M] = g
+    pad = pad.reshape(nq, stride, 4)
+    return (pad[:, :, 0] << 6 | pad[:, :, 1] << 4 | pad[:, :, 2] << 2 | pad[:, :, 3]).astype(np.uint8)
+
+
+def serving_queries(P: Panel, nq: int, seed: int, miss_lo=0.0, miss_hi=0.9):
+    """Populations and per-sample missingness thresholds of a query set drawn from
+    the panel's populations (missing fraction ~ U(miss_lo, miss_hi))."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    pop = rng.integers(0, P.npop, size=nq).astype(np.int32)
+    frac = rng.uniform(miss_lo, miss_hi, size=nq)
+    miss_thr = np.minimum(np.floor(frac * 4294967296.0), 4294967295.0).astype(np.uint32)
+    return pop, miss_thr
+
+
 def config_panel(cfg: int, nsnp=None, scale=1.0):
     """The five workload shapes (index 1..5); nsnp / scale shrink them for tests."""
     seed = 0xE16 + cfg

@@ -15,6 +15,7 @@
 #ifndef EIG_ABI_H
 #define EIG_ABI_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -51,6 +52,16 @@ const char *eig_last_error(const eig_ctx *ctx);
  * "tridiag", "syrk", ...), measured with HIP events on the context stream; -1 if
  * not measured. */
 double eig_last_ms(const eig_ctx *ctx, const char *what);
+
+/* Device and pinned host memory on the context's device, and a synchronous copy
+ * between any two of them (direction from the addresses).  Plumbing for callers
+ * that hold no other HIP runtime: query pools for the server, staging buffers.
+ * No reference counterpart (the reference has no device). */
+int eig_dev_alloc(eig_ctx *ctx, size_t bytes, void **ptr);
+int eig_dev_free(eig_ctx *ctx, void *ptr);
+int eig_host_alloc(eig_ctx *ctx, size_t bytes, void **ptr);
+int eig_host_free(eig_ctx *ctx, void *ptr);
+int eig_memcpy(eig_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 /* Copy packed genotype rows (GENO layout: nsnp records of rlen bytes, sample k in
  * byte k/4 bits 7-6,5-4,3-2,1-0) to the device.  Replaces inpack's in-memory
@@ -120,6 +131,59 @@ int eig_loadings(eig_ctx *ctx, const double *evecs, int32_t K, const uint8_t *ig
  * failed (previous individual's answer kept, as the reference does), 3 not used. */
 int eig_lsqproject(eig_ctx *ctx, const int32_t *use, double *acoeffs, double *bcoeffs,
                    int32_t *status);
+
+/* ---- lsqproject serving ------------------------------------------------------
+ * The reference projects samples once per run, inside lsqproj (smartpca.c:
+ * 4606-4757); the server keeps that step's inputs (loadings, fxscal, fancy, xmean,
+ * kept-SNP mask, eigscale) resident and answers projection requests as they
+ * arrive.  Query rows are sample-major: SNP m of a sample at byte m/4, bits 7-6,
+ * 5-4, 3-2, 1-0 for m%4 = 0..3 (the GENO bit order), code 3 = missing, over the
+ * same nsnp SNPs as the panel.  Per sample the result is the reference's
+ * lsqproj + regressit + solvit over its observed kept SNPs, times eigscale
+ * (smartpca.c:1556-1564).  status per sample: 0 solved, 1 ignored (<= K observed
+ * SNPs; coordinates 0), 2 Cholesky failed (the previous sample of the same
+ * request's answer, as regsubs.c:51-53).  A sample's result depends only on its
+ * own row: it is the same alone and batched with any other requests. */
+#define EIG_PENDING 1 /* eig_server_poll: not finished yet */
+typedef struct eig_server eig_server;
+
+/* Start a server on the context's device from its current loadings (after
+ * eig_loadings with K eigenvectors).  eigscale: K factors (NULL = 1).  nlanes:
+ * batches in flight at once (0 = 4, at most 8); max_batch: samples per batch
+ * (0 = 1024, at most 1024) -- together they bound the device memory admitted
+ * requests may use.  chunk: SNPs per partial sum (multiple of 64; 0 = chosen by
+ * the built-in launch-shape tuner at start-up).  The context may be used or
+ * closed afterwards; the server owns its copies. */
+int eig_server_start(eig_ctx *ctx, const double *eigscale, int32_t nlanes, int32_t max_batch,
+                     int32_t chunk, eig_server **out);
+/* Queue nq (1..max_batch) query rows of `stride` bytes.  on_device != 0: rows is
+ * device memory (16-byte aligned, stride a multiple of 16 and >= the padded row
+ * size eig_server_info reports); it is read in place and must stay valid and
+ * unchanged until the request is polled.  on_device == 0: host memory (stride >=
+ * ceil(nsnp/4)), copied to the device when the request is admitted and read
+ * until then.  Thread-safe. */
+int eig_server_submit(eig_server *s, const uint8_t *rows, int64_t stride, int32_t nq,
+                      int32_t on_device, int64_t *ticket);
+/* Result of a ticket: EIG_OK with coords (nq x K, row per sample), status (nq) and
+ * the submit-to-completion latency; EIG_PENDING if not done and wait == 0 (wait
+ * != 0 blocks).  A finished ticket is released by the poll that returns it. */
+int eig_server_poll(eig_server *s, int64_t ticket, int32_t wait, double *coords,
+                    int32_t *status, double *latency_ms);
+/* Shape and counters: K, nsnp, padded device row bytes, the chunk and persistent
+ * grid in use, batches launched and samples admitted so far.  Any may be NULL. */
+int eig_server_info(eig_server *s, int32_t *K, int64_t *nsnp, int64_t *rowpad, int32_t *chunk,
+                    int32_t *grid, int64_t *batches, int64_t *samples);
+const char *eig_server_error(const eig_server *s);
+/* Finish queued and running requests, then free the server. */
+void eig_server_stop(eig_server *s);
+
+/* Seeded synthetic query rows written to device memory rows_dev (nq rows of
+ * `stride` bytes, sample-major as above): sample q draws both allele copies from
+ * population pop[q] (thr as for eig_synth_packed) and is missing at a SNP when a
+ * draw falls below miss_thr[q].  Restated in eig_amd/synth.py for CPU checks. */
+int eig_synth_queries(eig_ctx *ctx, int64_t nsnp, int32_t nq, int32_t npop, const uint32_t *thr,
+                      const int32_t *pop, const uint32_t *miss_thr, uint64_t seed,
+                      uint8_t *rows_dev, int64_t stride);
 
 #ifdef __cplusplus
 }
