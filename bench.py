@@ -174,6 +174,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--forms", default="bf16x4,bf16x3,fp64",
+                    help="forms to run; anything but all three prints stage times only "
+                         "(for profiler runs)")
+    ap.add_argument("--copy-calib-mb", type=int, default=0,
+                    help="first copy this many MiB device-to-device (the known-size copy "
+                         "that scales the HBM byte counters of a --pmc run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported",
@@ -188,9 +194,15 @@ def main():
     eng = eig_amd.Engine(0)
     P.to_device(eng)
     xindex = np.arange(N, dtype=np.int32)
+    if args.copy_calib_mb:
+        nb = args.copy_calib_mb << 20
+        a, b = eng.dev_alloc(nb), eng.dev_alloc(nb)
+        eng.memcpy(b, a, nb)
+        eng.dev_free(a)
+        eng.dev_free(b)
     res = {}
-    forms = ((eig_amd.FORM_BF16X4, "bf16x4"), (eig_amd.FORM_BF16X3, "bf16x3"),
-             (eig_amd.FORM_FP64, "fp64"))
+    forms = [(f, n) for f, n in ((eig_amd.FORM_BF16X4, "bf16x4"), (eig_amd.FORM_BF16X3, "bf16x3"),
+                                 (eig_amd.FORM_FP64, "fp64")) if n in args.forms.split(",")]
     for form, name in forms:
         for _ in range(args.warmup):
             one_pass(eng, xindex, form, nvec)
@@ -208,6 +220,11 @@ def main():
                          stats_ms=eng.last_ms("snpstats"), select_ms=eng.last_ms("select"),
                          parts={k: eng.last_ms(k) for k in ("tridiag", "bisect", "invit", "eigvec",
                                                           "xtx_finalize")})
+    if len(res) != 3:
+        print(json.dumps({"forms": {k: {"s": v["t"], "syrk_ms": v["syrk_ms"], "eig_ms": v["eig_ms"],
+                                        **v["parts"]} for k, v in res.items()}}))
+        eng.close()
+        return
     b, b3, f = res["bf16x4"], res["bf16x3"], res["fp64"]
     top = slice(0, nvec)
 

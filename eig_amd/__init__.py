@@ -45,6 +45,7 @@ SIGNATURES = {
     "eig_eigh": (C.c_int, [_vp, _f64p, C.c_int32, C.c_int32, _f64p, _vp]),
     "eig_loadings": (C.c_int, [_vp, _f64p, C.c_int32, _vp, _vp, _vp]),
     "eig_lsqproject": (C.c_int, [_vp, _i32p, _f64p, _f64p, _i32p]),
+    "eig_fpca": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, _vp, _f64p, _vp]),
     "eig_dev_alloc": (C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
     "eig_dev_free": (C.c_int, [_vp, _vp]),
     "eig_host_alloc": (C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
@@ -216,6 +217,14 @@ class Engine:
         self._chk(self._L.eig_loadings(self.h, ev.reshape(-1), K, _ptr(ign), _ptr(ff), _ptr(fx)),
                   "eig_loadings")
         return ff, fx
+
+    def fpca(self, K, L, I, seed=0, omega=None):
+        """fastmode randomized PCA over the selected samples (after snpstats)."""
+        om = None if omega is None else np.ascontiguousarray(omega, dtype=np.float64)
+        ev = np.zeros(K)
+        vec = np.zeros((K, self.nrows))
+        self._chk(self._L.eig_fpca(self.h, K, L, I, seed, _ptr(om), ev, _ptr(vec)), "eig_fpca")
+        return ev, vec
 
     def lsqproject(self, use, K):
         use = np.ascontiguousarray(use, dtype=np.int32)

@@ -29,7 +29,7 @@ UNITS = {
     "eigh.hip": True,
     "project.hip": False,
     "serve.hip": False,
-    "fpca.hip": True,
+    "fpca.hip": False,
 }
 
 COMMON = [
@@ -66,7 +66,8 @@ def build(verbose: bool = False) -> str:
             if verbose and out.strip():
                 print(out)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs +
+         ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     build_cli()
     return LIB

@@ -1,0 +1,723 @@
+// fastmode: randomized block-Krylov PCA of the normalised genotype matrix
+// (kjg_fpca, ksrc/kjg_fpca.c:24-101, with setgval's normalisation, gval.c:31-87).
+//
+// X is the ncols x nrows matrix of SNP rows, X[m][i] = ((g - mean) * fancy) / sqrt(2)
+// with mean = xmean / xfancy exactly as setgval recomputes it, and 0 for a missing
+// genotype.  Omega (nrows x L) is a Gaussian panel; for i < I
+//     Q_i = X Omega_i,   Omega_{i+1} = X^T Q_i / ncols,
+// Q_I = X Omega_I; the columns of Q = [Q_0 .. Q_I] are orthonormalised (the
+// reference takes the SVD of Q; here two passes of Gram + Jacobi eigen-
+// decomposition with rank truncation give an orthonormal basis U of the same
+// range); then B = X^T U, and the top-K left singular vectors of B (from the
+// eigen-decomposition of B^T B) are the eigenvectors, S^2 / ncols the eigenvalues
+// (kjg_fpca.c:79-95).
+//
+// The two panel products read X straight from the 2-bit sample-tile layout and
+// multiply on FP64 MFMA (16x16x4); both are persistent kernels whose grid and
+// split-K factor are picked by a timing tuner on the first call for a shape, and
+// the 2I+1 products of the power iteration are replayed as one captured graph.
+// The tall-skinny Gram and basis products are plain GEMMs (rocBLAS).
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "eig_internal.h"
+
+typedef __attribute__((__vector_size__(4 * sizeof(double)))) double f64x4;
+
+namespace {
+constexpr int kKC = 32;       // samples (X A) or SNPs (X^T B) per LDS stage
+constexpr int kMaxCT = 16;    // column tiles of 16 -> at most 256 columns
+constexpr int kJacMax = 256;  // order limit of the one-block Jacobi eigensolver
+}  // namespace
+
+// ---------------------------------------------------------------- levels
+// lf[m][g] for g = 0,1,2 and lf[m][3] = 0 (missing); SNPs not live or with no
+// observed genotype get zero rows.  Built without contraction (gval.c:73-80).
+__global__ void k_fpca_levels(const double *__restrict__ mean, const double *__restrict__ fancy,
+                              const int32_t *__restrict__ status, int64_t M,
+                              double *__restrict__ lf) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  double *o = lf + 4 * m;
+  const int32_t st = status[m];
+  if (st == EIG_SNP_DEAD || st == EIG_SNP_ALLMISS) {
+    o[0] = o[1] = o[2] = o[3] = 0.0;
+    return;
+  }
+  const double f = fancy[m];
+  const double xm = mean[m] * f;  // xmean[col] = pmean * yfancy (smartpca.c:3205)
+  const double mu = xm / f;       // mean = xmean[col] / xfancy[col] (gval.c:74)
+  const double r2 = sqrt(2.0);
+  for (int k = 0; k < 3; ++k) {
+    double y = (double)k - mu;
+    y *= f;
+    o[k] = y / r2;
+  }
+  o[3] = 0.0;
+}
+
+// ---------------------------------------------------------------- Gaussian panel
+__device__ __host__ inline uint64_t fp_mix64(uint64_t x) {
+  x ^= x >> 30;
+  x *= 0xBF58476D1CE4E5B9ull;
+  x ^= x >> 27;
+  x *= 0x94D049BB133111EBull;
+  x ^= x >> 31;
+  return x;
+}
+
+// Omega[i][c] ~ N(0,1) for i < n, c < L (Box-Muller on a counter hash); zero padding.
+__global__ void k_fpca_gauss(double *__restrict__ A, int32_t npad, int32_t lda, int32_t n,
+                             int32_t L, uint64_t seed) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)npad * lda) return;
+  const int32_t i = (int32_t)(idx / lda), c = (int32_t)(idx % lda);
+  double z = 0.0;
+  if (i < n && c < L) {
+    const uint64_t key = (uint64_t)i * (uint64_t)L + (uint64_t)c;
+    const uint64_t h1 = fp_mix64(seed + (key + 1) * 0x9E3779B97F4A7C15ull);
+    const uint64_t h2 = fp_mix64(h1 ^ 0xD1B54A32D192ED03ull);
+    const double u1 = ((double)(h1 >> 11) + 1.0) * 0x1.0p-53;  // (0, 1]
+    const double u2 = (double)(h2 >> 11) * 0x1.0p-53;          // [0, 1)
+    z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+  }
+  A[idx] = z;
+}
+
+// ---------------------------------------------------------------- Y = X A
+// Persistent over SNP tiles of 128 (4 waves x 32 SNPs); each wave holds a
+// 32 x 16CT FP64 accumulator.  A: npad x lda (rows = samples, zero padded);
+// Y rows m, columns [0, C) at Y[m * ldy + c].
+template <int CT>
+__global__ __launch_bounds__(256) void k_fpca_xa(const uint32_t *__restrict__ sel, int64_t M,
+                                                 int32_t nst, const double *__restrict__ lf,
+                                                 const double *__restrict__ A, int32_t lda,
+                                                 double *__restrict__ Y, int64_t ldy, int32_t C) {
+  __shared__ double sA[kKC][16 * CT];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t ntiles = (M + 127) / 128;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t mb = tile * 128 + 32 * w;
+    int64_t r[2];
+    double lv[2][3];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      r[t] = mb + 16 * t + (lane & 15);
+      const bool ok = r[t] < M;
+#pragma unroll
+      for (int g = 0; g < 3; ++g) lv[t][g] = ok ? lf[4 * r[t] + g] : 0.0;
+    }
+    f64x4 acc[2][CT];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int c = 0; c < CT; ++c) acc[t][c] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int32_t st = 0; st < nst; ++st) {
+      for (int kc = 0; kc < kTileS; kc += kKC) {
+        __syncthreads();
+        const double *src = A + (int64_t)(st * kTileS + kc) * lda;
+        for (int idx = threadIdx.x; idx < kKC * 16 * CT; idx += 256) {
+          const int k = idx / (16 * CT), c = idx % (16 * CT);
+          sA[k][c] = src[(int64_t)k * lda + c];
+        }
+        __syncthreads();
+        uint32_t wd[2][2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          if (r[t] < M) {
+            const uint2 v = *(const uint2 *)(sel + ((int64_t)st * M + r[t]) * kTileWords + kc / 16);
+            wd[t][0] = v.x;
+            wd[t][1] = v.y;
+          } else {
+            wd[t][0] = wd[t][1] = ~0u;
+          }
+        }
+#pragma unroll
+        for (int ks = 0; ks < kKC / 4; ++ks) {
+          const int sh = 2 * ((ks & 3) * 4 + (lane >> 4));
+          double a[2];
+#pragma unroll
+          for (int t = 0; t < 2; ++t) {
+            const uint32_t code = (wd[t][ks >> 2] >> sh) & 3u;
+            a[t] = code == 0 ? lv[t][0] : (code == 1 ? lv[t][1] : (code == 2 ? lv[t][2] : 0.0));
+          }
+#pragma unroll
+          for (int c = 0; c < CT; ++c) {
+            const double b = sA[4 * ks + (lane >> 4)][16 * c + (lane & 15)];
+            acc[0][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b, acc[0][c], 0, 0, 0);
+            acc[1][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1], b, acc[1][c], 0, 0, 0);
+          }
+        }
+      }
+    }
+    // accumulator layout: lane l, element e -> row l/16 + 4e, column l%16
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int c = 0; c < CT; ++c)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int64_t row = mb + 16 * t + (lane >> 4) + 4 * e;
+          const int col = 16 * c + (lane & 15);
+          if (row < M && col < C) Y[row * ldy + col] = acc[t][c][e];
+        }
+  }
+}
+
+// ---------------------------------------------------------------- partial X^T B
+// Persistent over (sample tile, SNP split) items; a block covers one sample tile
+// (4 waves x 32 samples) and SNPs [s*mchunk, min(M, (s+1)*mchunk)).  B: rows m,
+// columns [0, C) at B[m * ldb + c] (columns >= C read as 0).  part[s][i][c],
+// i < nst*128, c < 16CT (row stride ldp).
+template <int CT>
+__global__ __launch_bounds__(256) void k_fpca_xtb(const uint32_t *__restrict__ sel, int64_t M,
+                                                  int32_t nst, const double *__restrict__ lf,
+                                                  const double *__restrict__ B, int64_t ldb,
+                                                  int32_t C, int32_t S, int64_t mchunk,
+                                                  double *__restrict__ part, int32_t ldp) {
+  __shared__ double sB[kKC][16 * CT];
+  __shared__ double sL[kKC][4];
+  __shared__ uint32_t sW[kKC][kTileWords];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t items = (int64_t)nst * S;
+  const int64_t npad = (int64_t)nst * kTileS;
+  for (int64_t it = blockIdx.x; it < items; it += gridDim.x) {
+    const int32_t st = (int32_t)(it % nst), s = (int32_t)(it / nst);
+    const int64_t m0 = (int64_t)s * mchunk, m1 = min(M, m0 + mchunk);
+    f64x4 acc[2][CT];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int c = 0; c < CT; ++c) acc[t][c] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int64_t mb = m0; mb < m1; mb += kKC) {
+      __syncthreads();
+      for (int idx = threadIdx.x; idx < kKC * 16 * CT; idx += 256) {
+        const int k = idx / (16 * CT), c = idx % (16 * CT);
+        const int64_t m = mb + k;
+        sB[k][c] = (m < m1 && c < C) ? B[m * ldb + c] : 0.0;
+      }
+      if (threadIdx.x < kKC * 4) {
+        const int k = threadIdx.x >> 2, g = threadIdx.x & 3;
+        const int64_t m = mb + k;
+        sL[k][g] = m < m1 ? lf[4 * m + g] : 0.0;
+      } else if (threadIdx.x < kKC * 4 + kKC * kTileWords / 2) {
+        const int q = threadIdx.x - kKC * 4;  // one uint2 of one SNP's tile record
+        const int k = q >> 2, wv = (q & 3) * 2;
+        const int64_t m = mb + k;
+        uint2 v = make_uint2(~0u, ~0u);
+        if (m < m1) v = *(const uint2 *)(sel + ((int64_t)st * M + m) * kTileWords + wv);
+        sW[k][wv] = v.x;
+        sW[k][wv + 1] = v.y;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int ks = 0; ks < kKC / 4; ++ks) {
+        const int k = 4 * ks + (lane >> 4);
+        double a[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          const int sl = 32 * w + 16 * t + (lane & 15);  // sample within the tile
+          const uint32_t code = (sW[k][sl >> 4] >> (2 * (sl & 15))) & 3u;
+          a[t] = sL[k][code];
+        }
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+          const double b = sB[k][16 * c + (lane & 15)];
+          acc[0][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b, acc[0][c], 0, 0, 0);
+          acc[1][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1], b, acc[1][c], 0, 0, 0);
+        }
+      }
+    }
+    double *out = part + ((int64_t)s * npad + (int64_t)st * kTileS) * ldp;
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int c = 0; c < CT; ++c)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int row = 32 * w + 16 * t + (lane >> 4) + 4 * e;
+          out[(int64_t)row * ldp + 16 * c + (lane & 15)] = acc[t][c][e];
+        }
+  }
+}
+
+// out[i][c] = scale * sum_s part[s][i][c] (splits in order); padding columns of
+// out beyond ldp are left alone.
+__global__ void k_fpca_reduce(const double *__restrict__ part, int32_t S, int64_t npad,
+                              int32_t ldp, double scale, double *__restrict__ out, int32_t ldo) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= npad * ldp) return;
+  const int64_t i = idx / ldp;
+  const int c = (int)(idx % ldp);
+  double s = 0.0;
+  for (int32_t k = 0; k < S; ++k) s += part[(int64_t)k * npad * ldp + idx];
+  out[i * ldo + c] = scale * s;
+}
+
+// ---------------------------------------------------------------- Jacobi
+// All eigenpairs of a symmetric n x n matrix (n <= 256, column-major, lda) by
+// cyclic two-sided Jacobi with round-robin pairing, in one block.  On exit the
+// diagonal of A holds the eigenvalues and V (n x n, ldv) their eigenvectors in
+// its columns (unsorted).  Stops after a sweep in which no pair had
+// |a_pq| > tol * sqrt(|a_pp a_qq|), or after max_sweeps.
+__global__ __launch_bounds__(1024) void k_jacobi(double *__restrict__ A, int32_t lda,
+                                                 double *__restrict__ V, int32_t ldv, int32_t n,
+                                                 int32_t max_sweeps, double tol,
+                                                 int32_t *__restrict__ sweeps_out) {
+  __shared__ int32_t idx[kJacMax];
+  __shared__ int32_t pp[kJacMax / 2], pq[kJacMax / 2];
+  __shared__ double cs[kJacMax / 2], sn[kJacMax / 2];
+  __shared__ int rotated;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int np = n + (n & 1);  // an odd order gets a dummy index n
+  const int half = np / 2;
+  for (int e = tid; e < n * n; e += nt) {
+    const int i = e % n, j = e / n;
+    V[(int64_t)j * ldv + i] = (i == j) ? 1.0 : 0.0;
+  }
+  int sweep = 0;
+  for (; sweep < max_sweeps; ++sweep) {
+    if (tid == 0) rotated = 0;
+    for (int e = tid; e < np; e += nt) idx[e] = e;
+    __syncthreads();
+    for (int round = 0; round < np - 1; ++round) {
+      for (int k = tid; k < half; k += nt) {
+        int p = idx[k], q = idx[np - 1 - k];
+        if (p > q) {
+          const int t = p;
+          p = q;
+          q = t;
+        }
+        double c = 1.0, s = 0.0;
+        if (q < n) {
+          const double apq = A[(int64_t)q * lda + p];
+          const double app = A[(int64_t)p * lda + p], aqq = A[(int64_t)q * lda + q];
+          if (fabs(apq) > tol * sqrt(fabs(app * aqq)) && apq != 0.0) {
+            const double tau = (aqq - app) / (2.0 * apq);
+            const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+            c = 1.0 / sqrt(1.0 + t * t);
+            s = t * c;
+            rotated = 1;
+          }
+        }
+        pp[k] = p;
+        pq[k] = q < n ? q : p;
+        cs[k] = c;
+        sn[k] = q < n ? s : 0.0;
+      }
+      __syncthreads();
+      // columns: A <- A J, V <- V J
+      for (int e = tid; e < n * half; e += nt) {
+        const int i = e % n, k = e / n;
+        const double c = cs[k], s = sn[k];
+        if (s == 0.0) continue;
+        const int p = pp[k], q = pq[k];
+        double *ap = A + (int64_t)p * lda + i, *aq = A + (int64_t)q * lda + i;
+        const double x = *ap, y = *aq;
+        *ap = c * x - s * y;
+        *aq = s * x + c * y;
+        double *vp = V + (int64_t)p * ldv + i, *vq = V + (int64_t)q * ldv + i;
+        const double u = *vp, v = *vq;
+        *vp = c * u - s * v;
+        *vq = s * u + c * v;
+      }
+      __syncthreads();
+      // rows: A <- J^T A
+      for (int e = tid; e < n * half; e += nt) {
+        const int j = e % n, k = e / n;
+        const double c = cs[k], s = sn[k];
+        if (s == 0.0) continue;
+        const int p = pp[k], q = pq[k];
+        double *ap = A + (int64_t)j * lda + p, *aq = A + (int64_t)j * lda + q;
+        const double x = *ap, y = *aq;
+        *ap = c * x - s * y;
+        *aq = s * x + c * y;
+      }
+      __syncthreads();
+      // exact zeros for the annihilated pairs
+      for (int k = tid; k < half; k += nt)
+        if (sn[k] != 0.0) {
+          A[(int64_t)pq[k] * lda + pp[k]] = 0.0;
+          A[(int64_t)pp[k] * lda + pq[k]] = 0.0;
+        }
+      // next pairing: keep idx[0], rotate idx[1..np-1] by one (np <= 256 < threads)
+      {
+        const int e = tid + 1;
+        int v = 0;
+        if (e < np) v = (e == 1) ? idx[np - 1] : idx[e - 1];
+        __syncthreads();
+        if (e < np) idx[e] = v;
+        __syncthreads();
+      }
+    }
+    if (!rotated) break;
+    __syncthreads();
+  }
+  if (tid == 0) *sweeps_out = sweep;
+}
+
+// ---------------------------------------------------------------- host side
+struct FpcaTune {
+  int64_t M = -1;
+  int32_t nst = -1, CT = -1;
+  int32_t grid = 0, S = 0;
+};
+
+struct FpcaState {
+  rocblas_handle blas = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  const void *gkey[4] = {nullptr, nullptr, nullptr, nullptr};
+  int64_t gshape[5] = {0, 0, 0, 0, 0};
+  FpcaTune xa[kMaxCT + 1], xtb[kMaxCT + 1];
+};
+static FpcaState g_fpca[16];
+
+#define FP_KCASE(n, ...) \
+  case n: {              \
+    constexpr int CT = n; \
+    __VA_ARGS__;         \
+  } break;
+#define FP_CTSWITCH(ctx, ct, ...)                                                            \
+  switch (ct) {                                                                              \
+    FP_KCASE(1, __VA_ARGS__) FP_KCASE(2, __VA_ARGS__) FP_KCASE(3, __VA_ARGS__)               \
+    FP_KCASE(4, __VA_ARGS__) FP_KCASE(5, __VA_ARGS__) FP_KCASE(6, __VA_ARGS__)               \
+    FP_KCASE(7, __VA_ARGS__) FP_KCASE(8, __VA_ARGS__) FP_KCASE(9, __VA_ARGS__)               \
+    FP_KCASE(10, __VA_ARGS__) FP_KCASE(11, __VA_ARGS__) FP_KCASE(12, __VA_ARGS__)            \
+    FP_KCASE(13, __VA_ARGS__) FP_KCASE(14, __VA_ARGS__) FP_KCASE(15, __VA_ARGS__)            \
+    FP_KCASE(16, __VA_ARGS__)                                                                \
+    default:                                                                                 \
+      return eig_set_err(ctx, EIG_E_ARG, "fpca: column count out of range");                 \
+  }
+
+#define FP_BLAS(ctx, call)                                                                   \
+  do {                                                                                       \
+    rocblas_status s_ = (call);                                                              \
+    if (s_ != rocblas_status_success)                                                        \
+      return eig_set_err((ctx), EIG_E_HIP, "%s:%d rocBLAS %s: %d", __FILE__, __LINE__, #call, \
+                         (int)s_);                                                           \
+  } while (0)
+
+namespace {
+struct Plan {
+  int64_t M;
+  int32_t nst, npad;
+  const uint32_t *sel;
+  const double *lf;
+};
+}  // namespace
+
+static int launch_xa(eig_ctx *ctx, const Plan &P, int CT, int grid, const double *A, int32_t lda,
+                     double *Y, int64_t ldy, int32_t C, hipStream_t st) {
+  FP_CTSWITCH(ctx, CT,
+              hipLaunchKernelGGL((k_fpca_xa<CT>), dim3(grid), dim3(256), 0, st, P.sel, P.M, P.nst,
+                                 P.lf, A, lda, Y, ldy, C));
+  return EIG_OK;
+}
+
+static int launch_xtb(eig_ctx *ctx, const Plan &P, int CT, int grid, int S, const double *B,
+                      int64_t ldb, int32_t C, double *part, double *out, int32_t ldo, double scale,
+                      hipStream_t st) {
+  const int64_t mchunk = ((P.M + S - 1) / S + kKC - 1) / kKC * kKC;
+  const int32_t ldp = 16 * CT;
+  FP_CTSWITCH(ctx, CT,
+              hipLaunchKernelGGL((k_fpca_xtb<CT>), dim3(grid), dim3(256), 0, st, P.sel, P.M, P.nst,
+                                 P.lf, B, ldb, C, S, mchunk, part, ldp));
+  const int64_t tot = (int64_t)P.npad * ldp;
+  hipLaunchKernelGGL(k_fpca_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                     (const double *)part, S, (int64_t)P.npad, ldp, scale, out, ldo);
+  return EIG_OK;
+}
+
+template <class Fn>
+static int time_launch(eig_ctx *ctx, Fn &&fn, float &ms) {
+  hipStream_t st = ctx->stream;
+  EIG_HIP(ctx, hipEventRecord(ctx->ev0, st));
+  int rc = fn();
+  if (rc != EIG_OK) return rc;
+  EIG_HIP(ctx, hipEventRecord(ctx->ev1, st));
+  EIG_HIP(ctx, hipEventSynchronize(ctx->ev1));
+  EIG_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  return EIG_OK;
+}
+
+// Timing tuner for X A: persistent grid for this shape (A: npad x 16CT, Y scratch).
+static int tune_xa(eig_ctx *ctx, FpcaState &F, const Plan &P, int CT, const double *A,
+                   int32_t lda, double *Y, int64_t ldy) {
+  FpcaTune &ta = F.xa[CT];
+  hipStream_t st = ctx->stream;
+  const int cu = ctx->num_cu;
+  if (ta.M != P.M || ta.nst != P.nst) {
+    float best = 1e30f;
+    for (int gm : {1, 2, 4}) {
+      const int grid = (int)std::min<int64_t>((int64_t)gm * cu, (P.M + 127) / 128);
+      float ms = 0.f;
+      EIG_TRY(time_launch(
+          ctx, [&] { return launch_xa(ctx, P, CT, grid, A, lda, Y, ldy, 16 * CT, st); }, ms));
+      if (ms < best) {
+        best = ms;
+        ta.grid = grid;
+      }
+    }
+    ta.M = P.M;
+    ta.nst = P.nst;
+  }
+  return EIG_OK;
+}
+
+// Timing tuner for X^T B: persistent grid and split count (B: M x >= 16CT, out scratch).
+static int tune_xtb(eig_ctx *ctx, FpcaState &F, const Plan &P, int CT, const double *B,
+                    int64_t ldb, double *part, int32_t Smax, double *out, int32_t ldo) {
+  FpcaTune &tb = F.xtb[CT];
+  hipStream_t st = ctx->stream;
+  const int cu = ctx->num_cu;
+  if (tb.M != P.M || tb.nst != P.nst) {
+    float best = 1e30f;
+    for (int S : {1, 2, 4, 8}) {
+      if (S > Smax) continue;
+      for (int gm : {1, 2, 4}) {
+        const int grid = (int)std::min<int64_t>((int64_t)gm * cu, (int64_t)P.nst * S);
+        float ms = 0.f;
+        EIG_TRY(time_launch(
+            ctx,
+            [&] { return launch_xtb(ctx, P, CT, grid, S, B, ldb, 16 * CT, part, out, ldo, 1.0, st); },
+            ms));
+        if (ms < best) {
+          best = ms;
+          tb.grid = grid;
+          tb.S = S;
+        }
+      }
+    }
+    tb.M = P.M;
+    tb.nst = P.nst;
+  }
+  return EIG_OK;
+}
+
+// Eigen-decomposition of the n x n symmetric device matrix G (column-major, ld n):
+// eigenvalues descending in w (host), eigenvectors in the columns of V (device,
+// ld n) in the same order.
+static int small_eigh(eig_ctx *ctx, double *G, int32_t n, double *V, std::vector<double> &w,
+                      double *work, int32_t *d_sweeps) {
+  if (n > kJacMax) return eig_set_err(ctx, EIG_E_ARG, "fpca: basis larger than %d", kJacMax);
+  hipStream_t st = ctx->stream;
+  hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(1024), 0, st, G, n, work, n, n, 30, 1e-15, d_sweeps);
+  EIG_HIP(ctx, hipGetLastError());
+  std::vector<double> a((size_t)n * n);
+  int32_t sweeps = 0;
+  EIG_HIP(ctx, hipMemcpyAsync(a.data(), G, sizeof(double) * n * n, hipMemcpyDeviceToHost, st));
+  EIG_HIP(ctx, hipMemcpyAsync(&sweeps, d_sweeps, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  EIG_HIP(ctx, hipStreamSynchronize(st));
+  if (sweeps >= 30) return eig_set_err(ctx, EIG_E_CONVERGE, "fpca: Jacobi did not converge");
+  std::vector<int32_t> ord(n);
+  for (int32_t i = 0; i < n; ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) {
+    return a[(size_t)x * n + x] > a[(size_t)y * n + y];
+  });
+  w.resize(n);
+  for (int32_t k = 0; k < n; ++k) {
+    w[k] = a[(size_t)ord[k] * n + ord[k]];
+    EIG_HIP(ctx, hipMemcpyAsync(V + (size_t)k * n, work + (size_t)ord[k] * n, sizeof(double) * n,
+                                hipMemcpyDeviceToDevice, st));
+  }
+  return EIG_OK;
+}
+
+// Columns j < r of V scaled by 1 / sqrt(w_j) (device, column-major ld n).
+__global__ void k_scale_cols(double *__restrict__ V, int32_t n, int32_t r,
+                             const double *__restrict__ s) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n * r) return;
+  V[idx] *= s[idx / n];
+}
+
+extern "C" int eig_fpca(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint64_t seed,
+                        const double *omega, double *evals, double *evecs) {
+  if (!ctx || K <= 0 || L <= K || I <= 0 || !evals)
+    return eig_set_err(ctx, EIG_E_ARG, "eig_fpca: need 0 < K < L and I > 0");
+  const int32_t Ctot = (I + 1) * L;
+  if (L > 16 * kMaxCT || Ctot > kJacMax)
+    return eig_set_err(ctx, EIG_E_ARG, "eig_fpca: L <= %d and (I+1)*L <= %d", 16 * kMaxCT,
+                       kJacMax);
+  if (!ctx->have_stats) return eig_set_err(ctx, EIG_E_STATE, "eig_fpca before eig_snpstats");
+  EIG_HIP(ctx, hipSetDevice(ctx->device));
+  FpcaState &F = g_fpca[ctx->device & 15];
+  hipStream_t st = ctx->stream;
+  const int64_t M = ctx->nsnp;
+  const int32_t n = ctx->nrows, nst = ctx->nst, npad = nst * kTileS;
+  // ncols: SNPs on the path (not removed by the caller's filters)
+  std::vector<int32_t> status(M);
+  EIG_HIP(ctx, hipMemcpy(status.data(), ctx->status.p, 4 * M, hipMemcpyDeviceToHost));
+  int64_t ncols = 0;
+  for (int64_t m = 0; m < M; ++m) ncols += status[m] != EIG_SNP_DEAD;
+  if (ncols == 0) return eig_set_err(ctx, EIG_E_ARG, "eig_fpca: no SNPs");
+
+  const int CTL = (L + 15) / 16, lda = 16 * CTL;
+  const int64_t ldq = ((Ctot + 15) / 16) * 16;
+  const int Smax = 8;
+  // workspace: lf | Omega x2 | Q | part | G | V | Vwork | U | B | small
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  size_t off = 0;
+  auto add = [&](size_t b) {
+    size_t o = off;
+    off += up(b);
+    return o;
+  };
+  const size_t o_lf = add(sizeof(double) * 4 * M);
+  const size_t o_a0 = add(sizeof(double) * npad * lda), o_a1 = add(sizeof(double) * npad * lda);
+  const size_t o_q = add(sizeof(double) * M * ldq);
+  const size_t o_u = add(sizeof(double) * M * ldq);
+  const size_t o_part = add(sizeof(double) * (size_t)Smax * npad * 16 * kMaxCT);
+  const size_t o_b = add(sizeof(double) * npad * ldq);
+  const size_t o_g = add(sizeof(double) * kJacMax * kJacMax);
+  const size_t o_v = add(sizeof(double) * kJacMax * kJacMax);
+  const size_t o_vw = add(sizeof(double) * kJacMax * kJacMax);
+  const size_t o_sm = add(sizeof(double) * (kJacMax + 64));
+  const size_t o_ev = add(sizeof(double) * (size_t)K * npad);
+  void *old = ctx->scratch2.p;
+  EIG_TRY(eig_ensure(ctx, ctx->scratch2, off));
+  char *base = (char *)ctx->scratch2.p;
+  double *lf = (double *)(base + o_lf), *A0 = (double *)(base + o_a0), *A1 = (double *)(base + o_a1);
+  double *Q = (double *)(base + o_q), *U = (double *)(base + o_u), *part = (double *)(base + o_part);
+  double *Bm = (double *)(base + o_b), *G = (double *)(base + o_g), *V = (double *)(base + o_v);
+  double *Vw = (double *)(base + o_vw), *sm = (double *)(base + o_sm), *Ev = (double *)(base + o_ev);
+  int32_t *d_sweeps = (int32_t *)(sm + kJacMax);
+
+  if (!F.blas) {
+    FP_BLAS(ctx, rocblas_create_handle(&F.blas));
+  }
+  FP_BLAS(ctx, rocblas_set_stream(F.blas, st));
+  FP_BLAS(ctx, rocblas_set_pointer_mode(F.blas, rocblas_pointer_mode_host));
+
+  eig_time_begin(ctx);
+  hipLaunchKernelGGL(k_fpca_levels, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, st,
+                     (const double *)ctx->mean.p, (const double *)ctx->fancy.p,
+                     (const int32_t *)ctx->status.p, M, lf);
+  if (omega) {
+    std::vector<double> h((size_t)npad * lda, 0.0);
+    for (int32_t i = 0; i < n; ++i)
+      for (int32_t c = 0; c < L; ++c) h[(size_t)i * lda + c] = omega[(size_t)i * L + c];
+    EIG_HIP(ctx, hipMemcpyAsync(A0, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, st));
+    EIG_HIP(ctx, hipStreamSynchronize(st));
+  } else {
+    const int64_t tot = (int64_t)npad * lda;
+    hipLaunchKernelGGL(k_fpca_gauss, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, A0,
+                       npad, lda, n, L, seed);
+  }
+  EIG_HIP(ctx, hipGetLastError());
+  Plan P{M, nst, npad, (const uint32_t *)ctx->sel.p, lf};
+  // the tuners write scratch outputs (U and Bm), never Q or the panels in use
+  EIG_TRY(tune_xa(ctx, F, P, CTL, A0, lda, U, ldq));
+  EIG_TRY(tune_xtb(ctx, F, P, CTL, U, ldq, part, Smax, Bm, lda));
+  // ---- power iterations (captured once per shape and buffer set, then replayed)
+  const void *key[4] = {ctx->sel.p, lf, Q, part};
+  const int64_t shape[5] = {M, nst, L, I, (int64_t)F.xa[CTL].grid * 1000 + F.xtb[CTL].S};
+  const bool same = F.gexec && std::equal(key, key + 4, F.gkey) &&
+                    std::equal(shape, shape + 5, F.gshape) && old == ctx->scratch2.p;
+  auto power = [&]() -> int {
+    double *cur = A0, *nxt = A1;
+    for (int32_t i = 0; i < I; ++i) {
+      EIG_TRY(launch_xa(ctx, P, CTL, F.xa[CTL].grid, cur, lda, Q + (size_t)i * L, ldq, L, st));
+      EIG_TRY(launch_xtb(ctx, P, CTL, F.xtb[CTL].grid, F.xtb[CTL].S, Q + (size_t)i * L, ldq, L,
+                         part, nxt, lda, 1.0 / (double)ncols, st));
+      std::swap(cur, nxt);
+    }
+    EIG_TRY(launch_xa(ctx, P, CTL, F.xa[CTL].grid, cur, lda, Q + (size_t)I * L, ldq, L, st));
+    return EIG_OK;
+  };
+  if (getenv("EIG_NO_GRAPH")) {
+    EIG_TRY(power());
+  } else {
+    if (!same) {
+      if (F.gexec) hipGraphExecDestroy(F.gexec);
+      F.gexec = nullptr;
+      hipGraph_t graph;
+      EIG_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      int rc = power();
+      hipError_t e = hipStreamEndCapture(st, &graph);
+      if (rc != EIG_OK) return rc;
+      if (e != hipSuccess) return eig_set_err(ctx, EIG_E_HIP, "fpca capture: %s", hipGetErrorString(e));
+      e = hipGraphInstantiate(&F.gexec, graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      if (e != hipSuccess) return eig_set_err(ctx, EIG_E_HIP, "fpca graph: %s", hipGetErrorString(e));
+      std::copy(key, key + 4, F.gkey);
+      std::copy(shape, shape + 5, F.gshape);
+    }
+    EIG_HIP(ctx, hipGraphLaunch(F.gexec, st));
+  }
+  EIG_HIP(ctx, hipGetLastError());
+  eig_time_end(ctx, "fpca_power");
+
+  // ---- orthonormal basis of range(Q): two passes of Gram + eigen, rank-truncated
+  eig_time_begin(ctx);
+  const double one = 1.0, zero = 0.0;
+  std::vector<double> w;
+  int32_t r = Ctot;
+  const double *src = Q;
+  for (int pass = 0; pass < 2; ++pass) {
+    // G = src^T src over M rows (row-major M x r, ld ldq == column-major r x M)
+    FP_BLAS(ctx, rocblas_dgemm(F.blas, rocblas_operation_none, rocblas_operation_transpose, r, r,
+                               (rocblas_int)M, &one, src, (rocblas_int)ldq, src, (rocblas_int)ldq,
+                               &zero, G, r));
+    EIG_TRY(small_eigh(ctx, G, r, V, w, Vw, d_sweeps));
+    const double wmax = w.empty() ? 0.0 : w[0];
+    if (!(wmax > 0.0)) return eig_set_err(ctx, EIG_E_NUMERIC, "fpca: zero Krylov basis");
+    int32_t keep = 0;
+    while (keep < r && w[keep] > wmax * 1e-13) ++keep;
+    std::vector<double> s(keep);
+    for (int32_t j = 0; j < keep; ++j) s[j] = 1.0 / sqrt(w[j]);
+    EIG_HIP(ctx, hipMemcpyAsync(sm, s.data(), sizeof(double) * keep, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_scale_cols, dim3((unsigned)(((int64_t)r * keep + 255) / 256)), dim3(256),
+                       0, st, V, r, keep, (const double *)sm);
+    // U (row-major M x keep) = src V_keep:  U^T = V_keep^T src^T
+    double *dst = (pass == 0) ? U : Q;
+    FP_BLAS(ctx, rocblas_dgemm(F.blas, rocblas_operation_transpose, rocblas_operation_none, keep,
+                               (rocblas_int)M, r, &one, V, r, src, (rocblas_int)ldq, &zero, dst,
+                               (rocblas_int)ldq));
+    EIG_HIP(ctx, hipStreamSynchronize(st));
+    src = dst;
+    r = keep;
+  }
+  if (r < K) return eig_set_err(ctx, EIG_E_NUMERIC, "fpca: Krylov basis rank %d < K", r);
+  // ---- B = X^T U  (npad x r, ld ldq), then the top-K left singular vectors of B
+  const int CTr = (r + 15) / 16;
+  EIG_TRY(tune_xtb(ctx, F, P, CTr, src, ldq, part, Smax, Bm, (int32_t)ldq));
+  EIG_TRY(launch_xtb(ctx, P, CTr, F.xtb[CTr].grid, F.xtb[CTr].S, src, ldq, r, part, Bm,
+                     (int32_t)ldq, 1.0, st));
+  FP_BLAS(ctx, rocblas_dgemm(F.blas, rocblas_operation_none, rocblas_operation_transpose, r, r,
+                             npad, &one, Bm, (rocblas_int)ldq, Bm, (rocblas_int)ldq, &zero, G, r));
+  EIG_TRY(small_eigh(ctx, G, r, V, w, Vw, d_sweeps));
+  std::vector<double> s(K);
+  for (int32_t j = 0; j < K; ++j) {
+    if (!(w[j] > 0.0)) return eig_set_err(ctx, EIG_E_NUMERIC, "fpca: singular value is zero");
+    s[j] = 1.0 / sqrt(w[j]);
+  }
+  EIG_HIP(ctx, hipMemcpyAsync(sm, s.data(), sizeof(double) * K, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_scale_cols, dim3((unsigned)(((int64_t)r * K + 255) / 256)), dim3(256), 0,
+                     st, V, r, K, (const double *)sm);
+  // Ev (column-major K x npad == row-major npad x K) = V_K^T B^T
+  FP_BLAS(ctx, rocblas_dgemm(F.blas, rocblas_operation_transpose, rocblas_operation_none, K, npad,
+                             r, &one, V, r, Bm, (rocblas_int)ldq, &zero, Ev, K));
+  std::vector<double> ev((size_t)npad * K);
+  EIG_HIP(ctx, hipMemcpyAsync(ev.data(), Ev, sizeof(double) * ev.size(), hipMemcpyDeviceToHost, st));
+  EIG_HIP(ctx, hipStreamSynchronize(st));
+  eig_time_end(ctx, "fpca_finish");
+  for (int32_t j = 0; j < K; ++j) {
+    evals[j] = w[j] / (double)ncols;  // S^2 / m (kjg_fpca.c:91-94)
+    if (!evecs) continue;
+    // unit norm, largest-|.| component positive
+    double nrm = 0.0, big = 0.0;
+    for (int32_t i = 0; i < n; ++i) {
+      const double x = ev[(size_t)i * K + j];
+      nrm += x * x;
+      if (fabs(x) > fabs(big)) big = x;
+    }
+    const double f = (big < 0.0 ? -1.0 : 1.0) / sqrt(nrm);
+    for (int32_t i = 0; i < n; ++i) evecs[(size_t)j * n + i] = f * ev[(size_t)i * K + j];
+  }
+  return EIG_OK;
+}

@@ -374,6 +374,61 @@ int main(int argc, char **argv) {
     outl = fopen(S.outliername, "w");
     if (!outl) die("can't open %s", S.outliername);
   }
+  /* ---- fastmode (smartpca.c:955-980): randomized PCA, .evec only, then stop */
+  if (S.fastmode && S.numeigs > 0) {
+    const int K = S.numeigs;
+    const int L = S.fastdim < 0 ? 2 * K : S.fastdim;  /* smartpca.c:650-656 */
+    const int I = S.fastiter < 0 ? K : S.fastiter;
+    if (K >= L || I == 0) exit(1);                     /* kjg_fpca.c:26-29 */
+    printf("seed: %ld\n", S.seed);
+    nrows = 0;
+    for (int32_t i = 0; i < N; ++i)
+      if (!inds[i].ignore && inds[i].aff) xindex[nrows++] = i;
+    for (int64_t m = 0; m < Mk; ++m) live[m] = 1;
+    printf("number of samples used: %d number of snps used: %lld\n", nrows, (long long)Mk);
+    double t0 = now_s();
+    CHK(eig_select(ctx, xindex, nrows));
+    CHK(eig_snpstats(ctx, live, S.altnormstyle, S.fancynorm, S.minallelecnt, S.maxmissing, NULL,
+                     NULL, NULL, NULL, NULL, NULL));
+    double *fev = malloc(sizeof(double) * K), *fvec = malloc(sizeof(double) * K * (size_t)nrows);
+    CHK(eig_fpca(ctx, K, L, I, (uint64_t)S.seed, NULL, fev, fvec));
+    if (S.verbose) printf("fastmode: %.3f s on the device\n", now_s() - t0);
+    /* printevecs (smartpca.c:3811-3875): header, then rows only when every
+       individual is an axis sample */
+    FILE *of = stdout;
+    if (S.outputname) {
+      of = fopen(S.outputname, "w");
+      if (!of) die("can't open %s", S.outputname);
+    }
+    fprintf(of, "%20s ", "#eigvals:");
+    for (int j = 0; j < K; ++j) fprintf(of, "%9.3f ", fev[j]);
+    fprintf(of, "\n");
+    if (nrows != N) {
+      if (of != stdout) fclose(of);
+      die("... not yet implemented!");
+    }
+    printf("writing eigenvecotrs for  %d samples\n", nrows);
+    for (int j = 0; j < K; ++j) { /* setfvecs (x10) then unit norm */
+      double *x = fvec + (size_t)j * nrows, y = 0.0;
+      for (int32_t i = 0; i < nrows; ++i) {
+        x[i] *= 10.0;
+        y += x[i] * x[i];
+      }
+      const double f = 1.0 / sqrt(y);
+      for (int32_t i = 0; i < nrows; ++i) x[i] *= f;
+    }
+    for (int32_t r = 0; r < nrows; ++r) {
+      const Ind *ip = &inds[xindex[r]];
+      fprintf(of, "%20s ", ip->id);
+      for (int j = 0; j < K; ++j)
+        fprintf(of, S.hiprec ? "%12.6f  " : "%10.4f  ", fvec[(size_t)j * nrows + r]);
+      fprintf(of, "%15s\n", ip->pop);
+    }
+    if (of != stdout) fclose(of);
+    printf("end of smartpca(fastmode)\n");
+    eig_close(ctx);
+    return 0;
+  }
   int numoutiter = S.numoutliter >= 1 ? S.numoutliter + 1 : 1;
   int omode = S.numoutliter >= 1 ? S.outliermode : 2;
   int nvec_pass = S.numeigs > S.numoutleigs ? S.numeigs : S.numoutleigs;

@@ -132,6 +132,18 @@ int eig_loadings(eig_ctx *ctx, const double *evecs, int32_t K, const uint8_t *ig
 int eig_lsqproject(eig_ctx *ctx, const int32_t *use, double *acoeffs, double *bcoeffs,
                    int32_t *status);
 
+/* fastmode: randomized block-Krylov PCA (kjg_fpca, ksrc/kjg_fpca.c:24-101) of the
+ * selected samples over every SNP not marked dead in the last eig_snpstats (the
+ * keep rule does not apply, as in the reference), normalised as setgval does
+ * (gval.c:31-87: ((g - xmean/xfancy) * xfancy) / sqrt(2), missing 0).  K
+ * eigenpairs from L = fastdim > K columns and I = fastiter > 0 iterations,
+ * (I+1)*L <= 256.  omega: nrows x L host Gaussian start panel, or NULL for the
+ * built-in generator seeded by `seed` (the reference draws it from GSL's default
+ * generator, kjg_gsl.c:96-186).  evals (K): S^2 / ncols (kjg_fpca.c:91-94);
+ * evecs (K x nrows, may be NULL): unit norm, largest-|.| component positive. */
+int eig_fpca(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint64_t seed, const double *omega,
+             double *evals, double *evecs);
+
 /* ---- lsqproject serving ------------------------------------------------------
  * The reference projects samples once per run, inside lsqproj (smartpca.c:
  * 4606-4757); the server keeps that step's inputs (loadings, fxscal, fancy, xmean,

@@ -168,6 +168,55 @@ def xtx_lookup(rows, xindex, levels, status, nthreads=8):
     return lower
 
 
+def fastmode_levels(mean, fancy, status):
+    """setgval's normalisation table (gval.c:66-80): for g = 0,1,2
+    ((g - xmean/xfancy) * xfancy) / sqrt(2) with xmean = mean*fancy
+    (getcolxz, smartpca.c:3205-3206); column 3 (missing) is 0.  SNPs with no
+    observed genotype (status 2) are all zero.  Returns float64 [M, 4]."""
+    mean = np.asarray(mean, dtype=np.float64)
+    fancy = np.asarray(fancy, dtype=np.float64)
+    lf = np.zeros((len(mean), 4))
+    ok = np.asarray(status) != 2
+    xm = mean[ok] * fancy[ok]
+    mu = xm / fancy[ok]
+    r2 = math.sqrt(2.0)
+    for k in range(3):
+        y = float(k) - mu
+        y = y * fancy[ok]
+        lf[ok, k] = y / r2
+    return lf
+
+
+def fastmode_matrix(rows, xindex, lf, nind):
+    """X [ncols, nrows]: X[m][i] = lf[m][g] (kjg_geno_get_normalized_row, gval.c:172-199)."""
+    g = decode_rows(rows, nind)[:, np.asarray(xindex)]
+    g = np.where(g < 0, 3, g)
+    return np.take_along_axis(lf, g, axis=1)
+
+
+def fpca(X, K, L, I, omega):
+    """kjg_fpca (ksrc/kjg_fpca.c:24-101) restated with numpy: X is ncols x nrows,
+    omega the nrows x L start panel (the reference draws it from GSL's default
+    generator, kjg_gsl.c:96-186).  Q_i = X G (XTXA, 104-128), G <- X^T Q_i / m
+    (gsl_matrix_scale at 52), Q_I = X G (XA, 130-153); Q <- left singular vectors
+    of Q (dgesvd jobu 'O', kjg_gsl.c:198-209); B = X^T Q (XTB, 155-178); the
+    eigenvectors are the first K left singular vectors of B and the eigenvalues
+    S^2 / m (79-95).  Returns (evals [K], evecs [K, nrows]) with the largest-|.|
+    component of each eigenvector made positive."""
+    m, n = X.shape
+    Q = np.zeros((m, (I + 1) * L))
+    G = np.array(omega, dtype=np.float64)
+    for i in range(I):
+        Qi = X @ G
+        Q[:, i * L:(i + 1) * L] = Qi
+        G = (X.T @ Qi) * (1.0 / m)
+    Q[:, I * L:] = X @ G
+    U = np.linalg.svd(Q, full_matrices=False)[0]
+    B = X.T @ U
+    UB, S = np.linalg.svd(B, full_matrices=False)[:2]
+    return S[:K] ** 2 / m, sign_rule(UB[:, :K].T)
+
+
 def finalize_xtx(lower, n):
     full = np.zeros((n, n))
     y = lib().orc_finalize_xtx(np.ascontiguousarray(lower), n, full.reshape(-1))

@@ -200,3 +200,31 @@ def test_cli_outlier_removal_matches_oracle(tmp_path):
                        atol=2e-3)
     ids = [l.split()[0] for l in (tmp_path / "out.evec").read_text().splitlines()[1:]]
     assert P.ids()[0][7] not in ids
+
+
+@pytest.mark.gpu
+def test_cli_fastmode_writes_the_randomized_pca(tmp_path):
+    """fastmode: YES -> .evec only (header eigenvalues S^2/ncols, unit-norm rows);
+    checked against the exact eigenpairs of X^T X / ncols that the randomized
+    method converges to (parity with the reference's GSL start panel is unpinned)."""
+    P, prefix = _synthetic(tmp_path, 4000, 0.1)
+    par = dict(genotypename=prefix + ".geno", snpname=prefix + ".snp",
+               indivname=prefix + ".ind", evecoutname="out.evec", evaloutname="out.eval",
+               numoutevec=2, fastmode="YES", fastiter=8, seed=77)
+    write_par(tmp_path / "par", **par)
+    print(f"[gpu-case] cli fastmode N={P.nind} M={P.nsnp}", flush=True)
+    r = run_cli(str(tmp_path / "par"), tmp_path)
+    assert "end of smartpca(fastmode)" in r.stdout
+    assert not (tmp_path / "out.eval").exists()
+    head, rows = _parse_evec((tmp_path / "out.evec").read_text())
+    rows_g = P.packed()
+    xindex = np.arange(P.nind, dtype=np.int32)
+    st = O.snpstats(rows_g, xindex)
+    lf = O.fastmode_levels(st["mean"], st["fancy"], st["status"])
+    X = O.fastmode_matrix(rows_g, xindex, lf, P.nind)
+    w, v = np.linalg.eigh(X.T @ X / X.shape[0])
+    w, v = w[::-1][:2], O.sign_rule(v[:, ::-1][:, :2].T)
+    assert np.allclose(head, w, atol=1.01e-3)
+    got = np.array([r_[1] for r_ in rows]).T
+    assert np.max(np.abs(got - v)) <= 1.01e-4 + 1e-6
+    assert [r_[0] for r_ in rows] == P.ids()[0]

@@ -240,3 +240,61 @@ def test_lsqproj_sparse_and_singular_rules():
     assert np.all(a[:, 2] == 0.0)
     # every solvable sample hits the singular Gram: regans stays at its zero start
     assert status[0] == 2 and np.all(a[:, 0] == 0.0)
+
+
+def _planted_panel(seed, M, npop, per, fst=0.1, miss=0.02):
+    rng = np.random.default_rng(seed)
+    p = rng.uniform(0.05, 0.5, size=M)
+    cols = []
+    for k in range(npop):
+        a = p * (1 - fst) / fst
+        b = (1 - p) * (1 - fst) / fst
+        pk = rng.beta(a, b)
+        cols.append(rng.binomial(2, pk[:, None], size=(M, per)))
+    g = np.concatenate(cols, axis=1).astype(np.int32)
+    g[rng.random(g.shape) < miss] = -1
+    return g
+
+
+def test_fastmode_levels_match_the_normal_mode_levels():
+    """setgval's table (through xmean/xfancy and /sqrt(2)) against the
+    column statistics' own levels (g - mean) * fancy: the same numbers up to the
+    rounding of the round trip (gval.c:73-80 vs smartpca.c:2282-2313)."""
+    g = _planted_panel(3, 400, 3, 30)
+    g[5, :] = -1  # no observed genotype
+    g[6, :] = 2   # monomorphic
+    rows = O.pack_genotypes(g)
+    xindex = np.arange(g.shape[1], dtype=np.int32)
+    for altnorm in (1, 0):
+        st = O.snpstats(rows, xindex, altnorm=altnorm)
+        lf = O.fastmode_levels(st["mean"], st["fancy"], st["status"])
+        assert np.all(lf[5] == 0) and np.all(lf[:, 3] == 0)
+        mean, fancy = st["mean"], st["fancy"]
+        for k in range(3):
+            ref = (k - mean) * fancy / np.sqrt(2.0)
+            ok = st["status"] != 2
+            assert np.allclose(lf[ok, k], ref[ok], rtol=1e-14, atol=1e-15)
+        assert lf[6, 2] == 0.0  # an all-2 SNP contributes nothing where observed
+
+
+def test_fpca_restatement_converges_to_the_exact_eigenpairs():
+    """kjg_fpca's randomized block Krylov method against the exact eigen-
+    decomposition of X^T X / m (its fixed point): top-K eigenvalues and
+    eigenvectors of a planted 4-population panel."""
+    g = _planted_panel(11, 3000, 4, 40)
+    rows = O.pack_genotypes(g)
+    n = g.shape[1]
+    xindex = np.arange(n, dtype=np.int32)
+    st = O.snpstats(rows, xindex)
+    lf = O.fastmode_levels(st["mean"], st["fancy"], st["status"])
+    X = O.fastmode_matrix(rows, xindex, lf, n)
+    K, L, I = 3, 6, 6
+    omega = np.random.default_rng(5).standard_normal((n, L))
+    ev, vec = O.fpca(X, K, L, I, omega)
+    w, v = np.linalg.eigh(X.T @ X / X.shape[0])
+    w, v = w[::-1][:K], O.sign_rule(v[:, ::-1][:, :K].T)
+    assert np.allclose(ev, w, rtol=1e-10)
+    assert np.max(np.abs(vec - v)) < 1e-7
+    # fewer iterations leave a visible gap: the check has teeth
+    ev1, _ = O.fpca(X, K, K + 1, 1, omega[:, :K + 1])
+    assert np.max(np.abs(ev1 - w) / w) > 1e-9
