@@ -5,18 +5,18 @@
 //     a padded leading dimension, so "column c below the diagonal" is row c right
 //     of the diagonal and every access is a contiguous 16-byte pair.  Two kernels
 //     run per column c (panel position j):
-//       k_symv : y = A_trail v with v rebuilt on the fly from the unscaled
-//                reflector column x and its scale; the panel dots p = V^T v,
-//                q = W^T v and y.v are reduced across blocks.
+//       k_symv : the reflector scalars of column c (dlarfg), then y = A_trail v
+//                with v rebuilt on the fly from the unscaled column x; per-block
+//                partial y.v;
 //       k_wcol : w = tau (y - V q - W p) + alpha v, where alpha = -tau/2 w.v is
 //                known up front from w.v = tau (y.v - 2 p.q); in the same pass the
-//                next row x = A[c+1] - (V W^T + W V^T)[c+1] and its norm; the last
-//                block forms the next reflector (dlarfg).
-//     Reductions across blocks are two-level and in fixed order (the last block
-//     of each group of 16, then the last group), so results do not depend on
-//     scheduling and no block ever waits for another.  After each panel k_syr2k
-//     applies A -= V W^T + W V^T to the trailing matrix.  The reflector of row c
-//     is left in A[c][c+1:], with tau[c].  The launch sequence for one n is
+//                next row x = A[c+1] - (V W^T + W V^T)[c+1] and per-block partials
+//                of its norm and of the next column's panel dots V^T x, W^T x.
+//     No block waits for another: each kernel reduces the previous kernel's
+//     per-block partials in block order (same bits in every block, independent of
+//     scheduling), so the sequence needs no atomics or fences.  After each panel
+//     k_syr2k applies A -= V W^T + W V^T to the trailing matrix.  The reflector of
+//     row c is left in A[c][c+1:], with tau[c].  The launch sequence for one n is
 //     captured once into a hipGraph and replayed.
 //  2. All eigenvalues of the tridiagonal T by 33-way multisection on Sturm counts
 //     (16 lanes per eigenvalue, two interleaved counts per lane).
@@ -36,13 +36,8 @@ constexpr int kMaxVec = 64;  // max eigenvectors computed
 constexpr int kGroup = 16;   // blocks per first-level reduction group
 
 enum {
-  S_TAU = 0,  // tau of the current column
-  S_SCALE,    // 1/(alpha-beta) of the current column
-  S_YV,       // y.v
-  S_GL, S_GU, S_PIVMIN, S_TNORM, S_NORM,
-  S_PD = 8,        // p_q = V_q . v  (kNB)
-  S_QD = 8 + kNB,  // q_q = W_q . v  (kNB)
-  S_COUNT = 8 + 2 * kNB
+  S_GL = 0, S_GU, S_PIVMIN, S_TNORM,
+  S_COUNT
 };
 
 struct EigWs {
@@ -53,7 +48,10 @@ struct EigWs {
   double *xtmp;   // current reflector column, unscaled (length lda, zero padded)
   double *y;
   double *sc;
-  double *part1, *part2;  // two-level reduction partials
+  double *part1, *part2;  // two-level reduction partials (back-transform)
+  double *pw;             // [2][maxw][kPW] column hand-off partials (by column parity)
+  double *ps;             // [maxblk] per-block y.v of k_symv
+  int32_t maxw;
   unsigned *ticket;       // [0] final, [1 + g] group counters
   double *eval;           // ascending
   double *X;              // nvec x n eigenvectors
@@ -123,22 +121,50 @@ __device__ inline double block_sum(double v, double *red) {
   return (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-// dlarfg for column c from alpha = x[c+1] and xn2 = sum_{r>c+1} x^2.
-__device__ inline void reflector_scalars(EigWs &ws, int32_t c, double alpha, double xn2) {
-  double tau = 0.0, scale = 0.0, beta = alpha;
+// ---- hand-off between the column kernels -----------------------------------
+// No kernel of the column sequence waits for another block: every cross-block sum
+// is written as per-block partials and reduced, in block order, by the blocks of
+// the next kernel (the kernel boundary makes the partials visible).
+//   k_wcol(c-1) -> pw[c & 1][b][kPW]: sum x^2 over rows >= c+2, x_{c+1} (alpha,
+//                  from block 0 only), and for q below column c's panel position
+//                  sum_{r >= c+2} V_q[r] x_r and sum W_q[r] x_r;
+//   k_symv(c)   -> ps[b]: sum over its rows of y_r v_r.
+constexpr int kPW = 2 + 2 * kNB;
+
+__device__ inline int32_t wcol_blocks(int32_t n, int32_t c) {  // producer blocks of column c
+  return c == 0 ? 1 : (n - c + 255) / 256;
+}
+
+// dlarfg scalars of column c from the producers' partials, reduced by one wave in
+// a fixed order (the same bits in every block that calls it).
+__device__ inline void col_scalars(const EigWs &ws, int32_t n, int32_t c, double &tau,
+                                   double &scale, double &beta) {
+  const double *pw = ws.pw + (int64_t)(c & 1) * ws.maxw * kPW;
+  const int32_t nw = wcol_blocks(n, c);
+  const int lane = threadIdx.x & 63;
+  double s0 = 0.0, s1 = 0.0;
+  for (int32_t b = lane; b < nw; b += 64) {
+    s0 += pw[(int64_t)b * kPW];
+    s1 += pw[(int64_t)b * kPW + 1];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    s0 += __shfl_down(s0, o, 64);
+    s1 += __shfl_down(s1, o, 64);
+  }
+  s0 = __shfl(s0, 0, 64);
+  s1 = __shfl(s1, 0, 64);
+  const double alpha = s1, xn2 = s0;
+  tau = 0.0;
+  scale = 0.0;
+  beta = alpha;
   if (xn2 > 0.0) {
     beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
     tau = (beta - alpha) / beta;
     scale = 1.0 / (alpha - beta);
   }
-  ws.e[c] = beta;
-  ws.e2[c] = beta * beta;
-  ws.tau[c] = tau;
-  ws.sc[S_TAU] = tau;
-  ws.sc[S_SCALE] = scale;
 }
 
-// column 0: x = row 0, d[0], reflector (one block)
+// column 0: x = row 0, d[0], and the partials of column 0 (one block)
 __global__ __launch_bounds__(256) void k_col0(EigWs ws, int32_t n) {
   __shared__ double red[4];
   double s = 0.0;
@@ -150,45 +176,82 @@ __global__ __launch_bounds__(256) void k_col0(EigWs ws, int32_t n) {
   s = block_sum(s, red);
   if (threadIdx.x == 0) {
     ws.d[0] = ws.a[0];
-    if (n > 1) reflector_scalars(ws, 0, ws.a[1], s);
+    ws.pw[0] = s;
+    ws.pw[1] = n > 1 ? ws.a[1] : 0.0;
   }
 }
 
 // ---------------------------------------------------------------- k_symv
 constexpr int kSymvRows = 8;  // rows per block
-constexpr int kSymvNV = 2 * kNB + 2;
 __global__ __launch_bounds__(256) void k_symv(EigWs ws, int32_t n, int32_t c, int32_t j) {
   __shared__ double red[4][kSymvRows];
-  __shared__ double vred[kSymvRows][kSymvNV];
-  __shared__ double mine[kSymvNV], tot[kSymvNV];
+  __shared__ double yv[kSymvRows];
+  __shared__ double sscale;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int32_t lda = ws.lda;
-  const double scale = ws.sc[S_SCALE];
+  if (wv == 0) {
+    double tau, scale, beta;
+    col_scalars(ws, n, c, tau, scale, beta);
+    if (lane == 0) {
+      sscale = scale;
+      if (blockIdx.x == 0) {
+        ws.e[c] = beta;
+        ws.e2[c] = beta * beta;
+        ws.tau[c] = tau;
+      }
+    }
+  }
+  __syncthreads();
+  const double scale = sscale;
   const int32_t r0 = c + 1 + blockIdx.x * kSymvRows;
   const int32_t a0 = (c + 1) & ~1;
   const int32_t npair = (((n + 1) & ~1) - a0) >> 1;
   const double2 *xp = (const double2 *)(ws.xtmp + a0);
+  // rows past n read row r0 instead (in bounds, result discarded), so every load
+  // in the loop is unconditional and all of them are in flight together
   const double2 *rp[kSymvRows];
-  bool ok[kSymvRows];
 #pragma unroll
-  for (int rr = 0; rr < kSymvRows; ++rr) {
-    ok[rr] = (r0 + rr) < n;
-    rp[rr] = (const double2 *)(ws.a + (int64_t)(ok[rr] ? r0 + rr : r0) * lda + a0);
-  }
+  for (int rr = 0; rr < kSymvRows; ++rr)
+    rp[rr] = (const double2 *)(ws.a + (int64_t)((r0 + rr) < n ? r0 + rr : r0) * lda + a0);
   double acc[kSymvRows];
 #pragma unroll
   for (int rr = 0; rr < kSymvRows; ++rr) acc[rr] = 0.0;
-  for (int32_t p = threadIdx.x; p < npair; p += 256) {
-    const double2 xx = xp[p];
-    const int32_t s0 = a0 + 2 * p;
-    // v: 0 up to column c, 1 at c+1, x*scale beyond (x is zero past n)
-    const double v0 = (s0 <= c) ? 0.0 : ((s0 == c + 1) ? 1.0 : xx.x * scale);
-    const double v1 = (s0 + 1 == c + 1) ? 1.0 : xx.y * scale;
-    double2 av[kSymvRows];
+  // v: 0 up to column c, 1 at c+1, x*scale beyond (x is zero past n); only the
+  // first pair (p = 0) can hold columns <= c+1
+  auto vpair = [&](const double2 xx, int32_t p, double &v0, double &v1) {
+    v0 = xx.x * scale;
+    v1 = xx.y * scale;
+    if (p == 0) {
+      const int32_t s0 = a0;
+      v0 = (s0 <= c) ? 0.0 : ((s0 == c + 1) ? 1.0 : v0);
+      v1 = (s0 + 1 == c + 1) ? 1.0 : v1;
+    }
+  };
+  int32_t p = threadIdx.x;
+  for (; p + 256 < npair; p += 512) {
+    const double2 x0 = xp[p], x1 = xp[p + 256];
+    double2 a0v[kSymvRows], a1v[kSymvRows];
 #pragma unroll
-    for (int rr = 0; rr < kSymvRows; ++rr) av[rr] = ok[rr] ? rp[rr][p] : make_double2(0.0, 0.0);
+    for (int rr = 0; rr < kSymvRows; ++rr) {
+      a0v[rr] = rp[rr][p];
+      a1v[rr] = rp[rr][p + 256];
+    }
+    double v0, v1, w0, w1;
+    vpair(x0, p, v0, v1);
+    vpair(x1, p + 256, w0, w1);
 #pragma unroll
-    for (int rr = 0; rr < kSymvRows; ++rr) acc[rr] += av[rr].x * v0 + av[rr].y * v1;
+    for (int rr = 0; rr < kSymvRows; ++rr)
+      acc[rr] += (a0v[rr].x * v0 + a0v[rr].y * v1) + (a1v[rr].x * w0 + a1v[rr].y * w1);
+  }
+  if (p < npair) {
+    const double2 x0 = xp[p];
+    double2 a0v[kSymvRows];
+#pragma unroll
+    for (int rr = 0; rr < kSymvRows; ++rr) a0v[rr] = rp[rr][p];
+    double v0, v1;
+    vpair(x0, p, v0, v1);
+#pragma unroll
+    for (int rr = 0; rr < kSymvRows; ++rr) acc[rr] += a0v[rr].x * v0 + a0v[rr].y * v1;
   }
 #pragma unroll
   for (int rr = 0; rr < kSymvRows; ++rr) {
@@ -197,58 +260,75 @@ __global__ __launch_bounds__(256) void k_symv(EigWs ws, int32_t n, int32_t c, in
     if (lane == 0) red[wv][rr] = a;
   }
   __syncthreads();
-  // per row: y, v into V[j] and A[c], and the row's share of p, q, y.v
-  for (int idx = threadIdx.x; idx < kSymvRows * kSymvNV; idx += 256) {
-    const int rr = idx / kSymvNV, q = idx % kSymvNV;
+  // per row: y, v into V[j] and A[c], and the row's y.v
+  if (threadIdx.x < kSymvRows) {
+    const int rr = threadIdx.x;
     const int32_t r = r0 + rr;
     double val = 0.0;
     if (r < n) {
       const double vr = (r == c + 1) ? 1.0 : ws.xtmp[r] * scale;
       const double yr = (red[0][rr] + red[1][rr]) + (red[2][rr] + red[3][rr]);
-      if (q == 2 * kNB) {
-        val = yr * vr;
-        ws.y[r] = yr;
-        ws.V[(int64_t)j * n + r] = vr;
-        ws.a[(int64_t)c * lda + r] = vr;
-      } else if (q < 2 * kNB && (q & (kNB - 1)) < j) {
-        const double *src = (q < kNB) ? ws.V : ws.W;
-        val = src[(int64_t)(q & (kNB - 1)) * n + r] * vr;
-      }
+      val = yr * vr;
+      ws.y[r] = yr;
+      ws.V[(int64_t)j * n + r] = vr;
+      ws.a[(int64_t)c * lda + r] = vr;
     }
-    vred[rr][q] = val;
+    yv[rr] = val;
   }
   __syncthreads();
-  if (threadIdx.x < kSymvNV) {
+  if (threadIdx.x == 0) {
     double sum = 0.0;
 #pragma unroll
-    for (int rr = 0; rr < kSymvRows; ++rr) sum += vred[rr][threadIdx.x];
-    mine[threadIdx.x] = sum;
+    for (int rr = 0; rr < kSymvRows; ++rr) sum += yv[rr];
+    ws.ps[blockIdx.x] = sum;
   }
-  __syncthreads();
-  if (!hreduce(ws, mine, kSymvNV, tot)) return;
-  if (threadIdx.x < 2 * kNB) ws.sc[S_PD + threadIdx.x] = tot[threadIdx.x];  // pd | qd
-  if (threadIdx.x == 0) ws.sc[S_YV] = tot[2 * kNB];
 }
 
 // ---------------------------------------------------------------- k_wcol
 __global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, int32_t j) {
   __shared__ double red[4];
-  __shared__ double pq[2 * kNB];
-  __shared__ double cc[2 * kNB];
-  __shared__ double mine[2], tot[2];
+  __shared__ double pq[2 * kNB];   // p_q = V_q . v, q_q = W_q . v (q < j)
+  __shared__ double cc[2 * kNB];   // V_q[c+1], W_q[c+1]
+  __shared__ double sh[3];         // tau, scale, y.v
+  __shared__ double wpart[4][kPW];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int32_t c1 = c + 1;
   const int32_t lda = ws.lda;
-  const double tau = ws.sc[S_TAU];
-  if (threadIdx.x < 2 * kNB) pq[threadIdx.x] = ws.sc[S_PD + threadIdx.x];  // pd | qd
+  const double *pwc = ws.pw + (int64_t)(c & 1) * ws.maxw * kPW;
+  const int32_t nw = wcol_blocks(n, c);
+  if (wv == 0) {
+    double tau, scale, beta;
+    col_scalars(ws, n, c, tau, scale, beta);
+    const int32_t gsym = (n - c - 1 + kSymvRows - 1) / kSymvRows;
+    double s = 0.0;
+    for (int32_t b = lane; b < gsym; b += 64) s += ws.ps[b];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if (lane == 0) {
+      sh[0] = tau;
+      sh[1] = scale;
+      sh[2] = s;
+    }
+  } else if (wv <= 2 && lane < j) {
+    // raw panel dots of column c: sum over the producer blocks, in block order
+    const int comp = (wv == 1 ? 2 : 2 + kNB) + lane;
+    double s = 0.0;
+    for (int32_t b = 0; b < nw; ++b) s += pwc[(int64_t)b * kPW + comp];
+    pq[(wv - 1) * kNB + lane] = s;
+  }
+  __syncthreads();
+  const double tau = sh[0], scale = sh[1];
   if (threadIdx.x < j) {
-    cc[threadIdx.x] = ws.V[(int64_t)threadIdx.x * n + c1];
-    cc[kNB + threadIdx.x] = ws.W[(int64_t)threadIdx.x * n + c1];
+    const int q = threadIdx.x;
+    cc[q] = ws.V[(int64_t)q * n + c1];
+    cc[kNB + q] = ws.W[(int64_t)q * n + c1];
+    pq[q] = cc[q] + scale * pq[q];                 // v = e_{c+1} + scale x beyond c+1
+    pq[kNB + q] = cc[kNB + q] + scale * pq[kNB + q];
   }
   __syncthreads();
   // alpha = -tau/2 w.v with w.v = tau (y.v - 2 sum_q p_q q_q)
   double spq = 0.0;
   for (int q = 0; q < j; ++q) spq += pq[q] * pq[kNB + q];
-  const double alpha = -0.5 * tau * (tau * (ws.sc[S_YV] - 2.0 * spq));
+  const double alpha = -0.5 * tau * (tau * (sh[2] - 2.0 * spq));
   // w_pre at row c+1 (same in every block)
   double wc1;
   {
@@ -257,7 +337,10 @@ __global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, in
     wc1 = tau * yy;
   }
   const int32_t r = c1 + blockIdx.x * 256 + threadIdx.x;
-  double nrm = 0.0;
+  // the next column (c+1) continues this panel: it needs V_q . x and W_q . x, q <= j
+  const int jn = ((c1 % kNB) != 0 && c1 + 1 < n) ? j + 1 : 0;
+  double x = 0.0, nrm = 0.0, al = 0.0;
+  bool live = false;
   if (r < n) {
     double yy = ws.y[r];
     double base = ws.a[(int64_t)c1 * lda + r];
@@ -270,20 +353,46 @@ __global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, in
     const double w = tau * yy;
     ws.W[(int64_t)j * n + r] = w + alpha * vr;
     // row c+1 minus panel column j:  V[j][c1] (=1) W[j][r] + W[j][c1] V[j][r]
-    const double x = base - w - wc1 * vr - 2.0 * alpha * vr;
+    x = base - w - wc1 * vr - 2.0 * alpha * vr;
     ws.xtmp[r] = x;
     if (r == c1) ws.d[c1] = x;
-    if (r >= c1 + 2) nrm = x * x;
-    // row c1+1 is thread 1 of block 0: its x is the next reflector's alpha and
-    // travels with the sums (the only value the finishing block needs)
-    if (r == c1 + 1) mine[1] = x;
+    live = r >= c1 + 2;
+    if (live) nrm = x * x;
+    if (r == c1 + 1) al = x;  // the next reflector's alpha (block 0 only)
   }
-  if (threadIdx.x == 0 && (blockIdx.x != 0 || c1 + 1 >= n)) mine[1] = 0.0;
-  double bs = block_sum(nrm, red);
-  if (threadIdx.x == 0) mine[0] = bs;
+  // this block's partials for column c+1, reduced per wave then across waves in order
+  {
+    double v0 = nrm, v1 = al;
+    for (int o = 32; o > 0; o >>= 1) {
+      v0 += __shfl_down(v0, o, 64);
+      v1 += __shfl_down(v1, o, 64);
+    }
+    if (lane == 0) {
+      wpart[wv][0] = v0;
+      wpart[wv][1] = v1;
+    }
+    const double xl = live ? x : 0.0;
+    for (int q = 0; q < jn; ++q) {
+      const double vq = live ? ws.V[(int64_t)q * n + r] : 0.0;
+      const double wq = live ? ws.W[(int64_t)q * n + r] : 0.0;
+      double a = vq * xl, b = wq * xl;
+      for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_down(a, o, 64);
+        b += __shfl_down(b, o, 64);
+      }
+      if (lane == 0) {
+        wpart[wv][2 + q] = a;
+        wpart[wv][2 + kNB + q] = b;
+      }
+    }
+  }
   __syncthreads();
-  if (!hreduce(ws, mine, 2, tot)) return;
-  if (threadIdx.x == 0 && c1 + 1 < n) reflector_scalars(ws, c1, tot[1], tot[0]);
+  double *pwn = ws.pw + (int64_t)(c1 & 1) * ws.maxw * kPW + (int64_t)blockIdx.x * kPW;
+  for (int t = threadIdx.x; t < kPW; t += 256) {
+    const int q = t < 2 ? 0 : ((t - 2) & (kNB - 1));
+    const bool use = t < 2 || q < jn;
+    pwn[t] = use ? (wpart[0][t] + wpart[1][t]) + (wpart[2][t] + wpart[3][t]) : 0.0;
+  }
 }
 
 // A[r][s] -= sum_q V[q][r] W[q][s] + W[q][r] V[q][s] for r, s >= k0 (64x64 tiles)
@@ -751,7 +860,8 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
   const int32_t lda = (int32_t)(((N + 7) / 8) * 8);
   const int64_t maxblk = (N + kSymvRows - 1) / kSymvRows + 8;
   const int64_t maxgrp = (maxblk + kGroup - 1) / kGroup + 2;
-  const int64_t maxnv = std::max<int64_t>(kSymvNV, (int64_t)kNB * kMaxVec);
+  const int64_t maxnv = (int64_t)kNB * kMaxVec;
+  const int64_t maxw = (N + 255) / 256 + 2;
   const int64_t npanel = (N + kNB - 1) / kNB + 1;
   size_t need = 0;
   auto add = [&](size_t cnt) {
@@ -764,7 +874,8 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
          o_sc = add(S_COUNT), o_p1 = add((size_t)maxblk * maxnv),
          o_p2 = add((size_t)maxgrp * maxnv), o_tk = add((size_t)maxgrp + 8), o_ev = add(N),
          o_X = add((size_t)kMaxVec * N), o_lu = add((size_t)kMaxVec * 5 * N),
-         o_Z = add(kNB * kMaxVec), o_T = add((size_t)npanel * kNB * kNB);
+         o_Z = add(kNB * kMaxVec), o_T = add((size_t)npanel * kNB * kNB),
+         o_pw = add((size_t)2 * maxw * kPW), o_ps = add((size_t)maxblk);
   void *old = ctx->ework.p;
   EIG_TRY(eig_ensure(ctx, ctx->ework, need));
   char *base = (char *)ctx->ework.p;
@@ -788,6 +899,9 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
   ws.lu = (double *)(base + o_lu);
   ws.Z = (double *)(base + o_Z);
   ws.Tall = (double *)(base + o_T);
+  ws.pw = (double *)(base + o_pw);
+  ws.ps = (double *)(base + o_ps);
+  ws.maxw = (int32_t)maxw;
   hipStream_t st = ctx->stream;
   if (lda != n) EIG_HIP(ctx, hipMemsetAsync(ws.a, 0, sizeof(double) * N * lda, st));
   EIG_HIP(ctx, hipMemcpy2DAsync(ws.a, sizeof(double) * lda, d_a, sizeof(double) * N,

@@ -6,11 +6,11 @@
 // genotype.  Omega (nrows x L) is a Gaussian panel; for i < I
 //     Q_i = X Omega_i,   Omega_{i+1} = X^T Q_i / ncols,
 // Q_I = X Omega_I; the columns of Q = [Q_0 .. Q_I] are orthonormalised (the
-// reference takes the SVD of Q; here two passes of Gram + Jacobi eigen-
-// decomposition with rank truncation give an orthonormal basis U of the same
-// range); then B = X^T U, and the top-K left singular vectors of B (from the
-// eigen-decomposition of B^T B) are the eigenvectors, S^2 / ncols the eigenvalues
-// (kjg_fpca.c:79-95).
+// reference takes the SVD of Q; here block Gram-Schmidt with reorthogonalisation,
+// each block orthonormalised by Gram + Jacobi eigendecomposition, gives an
+// orthonormal basis U of the same range); then B = X^T U, and the top-K left
+// singular vectors of B (from the eigen-decomposition of B^T B) are the
+// eigenvectors, S^2 / ncols the eigenvalues (kjg_fpca.c:79-95).
 //
 // The two panel products read X straight from the 2-bit sample-tile layout and
 // multiply on FP64 MFMA (16x16x4); both are persistent kernels whose grid and
@@ -577,6 +577,7 @@ extern "C" int eig_fpca(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint64_t 
   const size_t o_v = add(sizeof(double) * kJacMax * kJacMax);
   const size_t o_vw = add(sizeof(double) * kJacMax * kJacMax);
   const size_t o_sm = add(sizeof(double) * (kJacMax + 64));
+  const size_t o_cb = add(sizeof(double) * kJacMax * kJacMax);
   const size_t o_ev = add(sizeof(double) * (size_t)K * npad);
   void *old = ctx->scratch2.p;
   EIG_TRY(eig_ensure(ctx, ctx->scratch2, off));
@@ -586,6 +587,7 @@ extern "C" int eig_fpca(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint64_t 
   double *Bm = (double *)(base + o_b), *G = (double *)(base + o_g), *V = (double *)(base + o_v);
   double *Vw = (double *)(base + o_vw), *sm = (double *)(base + o_sm), *Ev = (double *)(base + o_ev);
   int32_t *d_sweeps = (int32_t *)(sm + kJacMax);
+  double *Cb = (double *)(base + o_cb);
 
   if (!F.blas) {
     FP_BLAS(ctx, rocblas_create_handle(&F.blas));
@@ -652,36 +654,71 @@ extern "C" int eig_fpca(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint64_t 
   EIG_HIP(ctx, hipGetLastError());
   eig_time_end(ctx, "fpca_power");
 
-  // ---- orthonormal basis of range(Q): two passes of Gram + eigen, rank-truncated
+  // ---- orthonormal basis U of range(Q), block by block (kjg_fpca takes the SVD of
+  // Q): each Krylov block is projected against the basis so far twice, orthonormalised
+  // from its column-scaled Gram matrix, projected once more and orthonormalised again.
+  // The blocks differ in scale by powers of the top eigenvalue, so one Gram matrix of
+  // all of Q would lose the small directions that the SVD keeps.
   eig_time_begin(ctx);
-  const double one = 1.0, zero = 0.0;
+  const double one = 1.0, zero = 0.0, mone = -1.0;
   std::vector<double> w;
-  int32_t r = Ctot;
-  const double *src = Q;
-  for (int pass = 0; pass < 2; ++pass) {
-    // G = src^T src over M rows (row-major M x r, ld ldq == column-major r x M)
-    FP_BLAS(ctx, rocblas_dgemm(F.blas, rocblas_operation_none, rocblas_operation_transpose, r, r,
-                               (rocblas_int)M, &one, src, (rocblas_int)ldq, src, (rocblas_int)ldq,
-                               &zero, G, r));
-    EIG_TRY(small_eigh(ctx, G, r, V, w, Vw, d_sweeps));
-    const double wmax = w.empty() ? 0.0 : w[0];
-    if (!(wmax > 0.0)) return eig_set_err(ctx, EIG_E_NUMERIC, "fpca: zero Krylov basis");
-    int32_t keep = 0;
-    while (keep < r && w[keep] > wmax * 1e-13) ++keep;
-    std::vector<double> s(keep);
-    for (int32_t j = 0; j < keep; ++j) s[j] = 1.0 / sqrt(w[j]);
-    EIG_HIP(ctx, hipMemcpyAsync(sm, s.data(), sizeof(double) * keep, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_scale_cols, dim3((unsigned)(((int64_t)r * keep + 255) / 256)), dim3(256),
-                       0, st, V, r, keep, (const double *)sm);
-    // U (row-major M x keep) = src V_keep:  U^T = V_keep^T src^T
-    double *dst = (pass == 0) ? U : Q;
-    FP_BLAS(ctx, rocblas_dgemm(F.blas, rocblas_operation_transpose, rocblas_operation_none, keep,
-                               (rocblas_int)M, r, &one, V, r, src, (rocblas_int)ldq, &zero, dst,
+  int32_t r = 0;
+  // W (nc columns, column-major nc x M at Wp, ld ldq) -= U_r (U_r^T W)
+  auto project = [&](double *Wp, int32_t nc) -> int {
+    if (r == 0) return EIG_OK;
+    FP_BLAS(ctx, rocblas_dgemm(F.blas, rocblas_operation_none, rocblas_operation_transpose, r, nc,
+                               (rocblas_int)M, &one, U, (rocblas_int)ldq, Wp, (rocblas_int)ldq,
+                               &zero, Cb, r));
+    FP_BLAS(ctx, rocblas_dgemm(F.blas, rocblas_operation_transpose, rocblas_operation_none, nc,
+                               (rocblas_int)M, r, &mone, Cb, r, U, (rocblas_int)ldq, &one, Wp,
                                (rocblas_int)ldq));
+    return EIG_OK;
+  };
+  // Out (keep columns) = orthonormal basis of the nc columns at Wp
+  auto orth = [&](const double *Wp, int32_t nc, double *Out, int32_t &keep) -> int {
+    keep = 0;
+    FP_BLAS(ctx, rocblas_dgemm(F.blas, rocblas_operation_none, rocblas_operation_transpose, nc, nc,
+                               (rocblas_int)M, &one, Wp, (rocblas_int)ldq, Wp, (rocblas_int)ldq,
+                               &zero, G, nc));
+    std::vector<double> g((size_t)nc * nc), dsc(nc);
+    EIG_HIP(ctx, hipMemcpyAsync(g.data(), G, sizeof(double) * g.size(), hipMemcpyDeviceToHost, st));
     EIG_HIP(ctx, hipStreamSynchronize(st));
-    src = dst;
-    r = keep;
+    for (int32_t i = 0; i < nc; ++i) {
+      const double gi = g[(size_t)i * nc + i];
+      dsc[i] = gi > 0.0 ? sqrt(gi) : 1.0;
+    }
+    for (int32_t jc = 0; jc < nc; ++jc)
+      for (int32_t i = 0; i < nc; ++i) g[(size_t)jc * nc + i] /= dsc[i] * dsc[jc];
+    EIG_HIP(ctx, hipMemcpyAsync(G, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice, st));
+    EIG_TRY(small_eigh(ctx, G, nc, V, w, Vw, d_sweeps));
+    if (!(w[0] > 0.0)) return EIG_OK;
+    while (keep < nc && w[keep] > 1e-13 * w[0]) ++keep;
+    std::vector<double> v((size_t)nc * keep);
+    EIG_HIP(ctx, hipMemcpyAsync(v.data(), V, sizeof(double) * v.size(), hipMemcpyDeviceToHost, st));
+    EIG_HIP(ctx, hipStreamSynchronize(st));
+    for (int32_t k = 0; k < keep; ++k)
+      for (int32_t i = 0; i < nc; ++i) v[(size_t)k * nc + i] /= sqrt(w[k]) * dsc[i];
+    EIG_HIP(ctx, hipMemcpyAsync(V, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, st));
+    FP_BLAS(ctx, rocblas_dgemm(F.blas, rocblas_operation_transpose, rocblas_operation_none, keep,
+                               (rocblas_int)M, nc, &one, V, nc, Wp, (rocblas_int)ldq, &zero, Out,
+                               (rocblas_int)ldq));
+    return EIG_OK;
+  };
+  for (int32_t blk = 0; blk <= I; ++blk) {
+    double *Ub = U + r, *Qb = Q + (size_t)blk * L;
+    EIG_HIP(ctx, hipMemcpy2DAsync(Ub, sizeof(double) * ldq, Qb, sizeof(double) * ldq,
+                                  sizeof(double) * L, (size_t)M, hipMemcpyDeviceToDevice, st));
+    EIG_TRY(project(Ub, L));
+    EIG_TRY(project(Ub, L));
+    int32_t k1 = 0, k2 = 0;
+    EIG_TRY(orth(Ub, L, Qb, k1));  // into the block's own (now free) columns of Q
+    if (k1 == 0) continue;
+    EIG_TRY(project(Qb, k1));
+    EIG_TRY(orth(Qb, k1, Ub, k2));
+    r += k2;
   }
+  EIG_HIP(ctx, hipStreamSynchronize(st));
+  const double *src = U;
   if (r < K) return eig_set_err(ctx, EIG_E_NUMERIC, "fpca: Krylov basis rank %d < K", r);
   // ---- B = X^T U  (npad x r, ld ldq), then the top-K left singular vectors of B
   const int CTr = (r + 15) / 16;

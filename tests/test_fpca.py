@@ -60,7 +60,9 @@ def test_fpca_matches_oracle_with_same_start_panel(eng):
     xindex = np.array([i for i in range(N) if i % 7 != 3], dtype=np.int32)
     live = np.ones(g.shape[0], np.uint8)
     live[100:160] = 0
-    K, L, I = 4, 8, 5
+    # four populations: three converged axes (an unconverged Ritz value in the noise
+    # bulk depends on rounding, in the reference's SVD as much as here)
+    K, L, I = 3, 8, 5
     omega = np.random.default_rng(9).standard_normal((len(xindex), L))
     _say(f"fpca vs oracle N={N} nrows={len(xindex)} M={g.shape[0]} K={K} L={L} I={I}")
     eng.load_packed(rows, N)
@@ -74,7 +76,7 @@ def test_fpca_matches_oracle_with_same_start_panel(eng):
     assert np.allclose(np.sum(vec * vec, axis=1), 1.0, atol=1e-13)
 
 
-@pytest.mark.parametrize("N,M,K,L,I", [(300, 20000, 10, 20, 10), (129, 3000, 2, 5, 3)])
+@pytest.mark.parametrize("N,M,K,L,I", [(300, 20000, 10, 20, 10), (129, 3000, 2, 6, 8)])
 def test_fpca_builtin_generator_reaches_exact_eigenpairs(eng, N, M, K, L, I):
     g = _panel(N, M, 12, N // 12 + 1)[:, :N]
     rows = O.pack_genotypes(g)
