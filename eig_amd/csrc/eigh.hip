@@ -44,7 +44,7 @@ struct EigWs {
   double *a;  // n x lda working matrix (destroyed); columns n..lda are zero
   int32_t lda;
   double *d, *e, *e2, *tau;
-  double *V, *W;  // kNB x n panel vectors
+  double *V, *W;  // n x kNB panel vectors (row-major)
   double *xtmp;   // current reflector column, unscaled (length lda, zero padded)
   double *y;
   double *sc;
@@ -270,7 +270,7 @@ __global__ __launch_bounds__(256) void k_symv(EigWs ws, int32_t n, int32_t c, in
       const double yr = (red[0][rr] + red[1][rr]) + (red[2][rr] + red[3][rr]);
       val = yr * vr;
       ws.y[r] = yr;
-      ws.V[(int64_t)j * n + r] = vr;
+      ws.V[(int64_t)r * kNB + j] = vr;
       ws.a[(int64_t)c * lda + r] = vr;
     }
     yv[rr] = val;
@@ -285,50 +285,59 @@ __global__ __launch_bounds__(256) void k_symv(EigWs ws, int32_t n, int32_t c, in
 }
 
 // ---------------------------------------------------------------- k_wcol
+// Panel vectors are stored row-major, V[r * kNB + q], so a thread loads its row's
+// coefficients in one batch; entries of columns not yet in this panel are stale
+// but always meet a zero coefficient.
 __global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, int32_t j) {
-  __shared__ double red[4];
-  __shared__ double pq[2 * kNB];   // p_q = V_q . v, q_q = W_q . v (q < j)
-  __shared__ double cc[2 * kNB];   // V_q[c+1], W_q[c+1]
-  __shared__ double sh[3];         // tau, scale, y.v
-  __shared__ double wpart[4][kPW];
+  __shared__ double red[4][4];
+  __shared__ double pqr[4][2 * kNB];  // raw panel dots per thread group
+  __shared__ double pq[2 * kNB];      // p_q = V_q . v, q_q = W_q . v (zero for q >= j)
+  __shared__ double cc[2 * kNB];      // V_q[c+1], W_q[c+1] (zero for q >= j)
+  __shared__ double sh[2];            // tau, scale
+  __shared__ double tb[256][17];      // transpose buffer of the block reduction
+  __shared__ double tb2[16][16];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int32_t c1 = c + 1;
   const int32_t lda = ws.lda;
   const double *pwc = ws.pw + (int64_t)(c & 1) * ws.maxw * kPW;
   const int32_t nw = wcol_blocks(n, c);
+  const int32_t gsym = (n - c - 1 + kSymvRows - 1) / kSymvRows;
+  // ---- prologue: every load of the three reductions is issued before any add
+  double yv = 0.0;
+  for (int32_t b = threadIdx.x; b < gsym; b += 256) yv += ws.ps[b];
+  double praw = 0.0;
+  {
+    const int comp = lane;  // 0..31 V dots, 32..63 W dots
+    const int q = comp & (kNB - 1);
+    if (q < j)
+      for (int32_t b = wv; b < nw; b += 4) praw += pwc[(int64_t)b * kPW + 2 + comp];
+  }
   if (wv == 0) {
     double tau, scale, beta;
     col_scalars(ws, n, c, tau, scale, beta);
-    const int32_t gsym = (n - c - 1 + kSymvRows - 1) / kSymvRows;
-    double s = 0.0;
-    for (int32_t b = lane; b < gsym; b += 64) s += ws.ps[b];
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
     if (lane == 0) {
       sh[0] = tau;
       sh[1] = scale;
-      sh[2] = s;
     }
-  } else if (wv <= 2 && lane < j) {
-    // raw panel dots of column c: sum over the producer blocks, in block order
-    const int comp = (wv == 1 ? 2 : 2 + kNB) + lane;
-    double s = 0.0;
-    for (int32_t b = 0; b < nw; ++b) s += pwc[(int64_t)b * kPW + comp];
-    pq[(wv - 1) * kNB + lane] = s;
   }
+  for (int o = 32; o > 0; o >>= 1) yv += __shfl_down(yv, o, 64);
+  if (lane == 0) red[wv][0] = yv;
+  pqr[wv][lane] = praw;
   __syncthreads();
   const double tau = sh[0], scale = sh[1];
-  if (threadIdx.x < j) {
-    const int q = threadIdx.x;
-    cc[q] = ws.V[(int64_t)q * n + c1];
-    cc[kNB + q] = ws.W[(int64_t)q * n + c1];
-    pq[q] = cc[q] + scale * pq[q];                 // v = e_{c+1} + scale x beyond c+1
-    pq[kNB + q] = cc[kNB + q] + scale * pq[kNB + q];
+  const double ydv = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
+  if (threadIdx.x < 2 * kNB) {
+    const int comp = threadIdx.x, q = comp & (kNB - 1);
+    const double raw = (pqr[0][comp] + pqr[1][comp]) + (pqr[2][comp] + pqr[3][comp]);
+    const double at = (comp < kNB) ? ws.V[(int64_t)c1 * kNB + q] : ws.W[(int64_t)c1 * kNB + q];
+    cc[comp] = q < j ? at : 0.0;
+    pq[comp] = q < j ? at + scale * raw : 0.0;  // v = e_{c+1} + scale x beyond c+1
   }
   __syncthreads();
   // alpha = -tau/2 w.v with w.v = tau (y.v - 2 sum_q p_q q_q)
   double spq = 0.0;
   for (int q = 0; q < j; ++q) spq += pq[q] * pq[kNB + q];
-  const double alpha = -0.5 * tau * (tau * (sh[2] - 2.0 * spq));
+  const double alpha = -0.5 * tau * (tau * (ydv - 2.0 * spq));
   // w_pre at row c+1 (same in every block)
   double wc1;
   {
@@ -337,61 +346,93 @@ __global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, in
     wc1 = tau * yy;
   }
   const int32_t r = c1 + blockIdx.x * 256 + threadIdx.x;
-  // the next column (c+1) continues this panel: it needs V_q . x and W_q . x, q <= j
-  const int jn = ((c1 % kNB) != 0 && c1 + 1 < n) ? j + 1 : 0;
-  double x = 0.0, nrm = 0.0, al = 0.0;
-  bool live = false;
-  if (r < n) {
-    double yy = ws.y[r];
-    double base = ws.a[(int64_t)c1 * lda + r];
-    for (int q = 0; q < j; ++q) {
-      const double vq = ws.V[(int64_t)q * n + r], wq = ws.W[(int64_t)q * n + r];
-      yy -= vq * pq[kNB + q] + wq * pq[q];
-      base -= cc[q] * wq + cc[kNB + q] * vq;
+  const bool inr = r < n;
+  const int32_t rs = inr ? r : c1;  // rows past n read row c+1 (in bounds, discarded)
+  double vrow[kNB], wrow[kNB];
+  {
+    const double2 *vp = (const double2 *)(ws.V + (int64_t)rs * kNB);
+    const double2 *wp = (const double2 *)(ws.W + (int64_t)rs * kNB);
+#pragma unroll
+    for (int h = 0; h < kNB / 2; ++h) {
+      const double2 a2 = vp[h], b2 = wp[h];
+      vrow[2 * h] = a2.x;
+      vrow[2 * h + 1] = a2.y;
+      wrow[2 * h] = b2.x;
+      wrow[2 * h + 1] = b2.y;
     }
-    const double vr = ws.V[(int64_t)j * n + r];
-    const double w = tau * yy;
-    ws.W[(int64_t)j * n + r] = w + alpha * vr;
-    // row c+1 minus panel column j:  V[j][c1] (=1) W[j][r] + W[j][c1] V[j][r]
-    x = base - w - wc1 * vr - 2.0 * alpha * vr;
+  }
+  double yy = ws.y[rs];
+  double base = ws.a[(int64_t)c1 * lda + rs];
+#pragma unroll
+  for (int q = 0; q < kNB; ++q) {
+    yy -= vrow[q] * pq[kNB + q] + wrow[q] * pq[q];
+    base -= cc[q] * wrow[q] + cc[kNB + q] * vrow[q];
+  }
+  // vrow[j]: this column's v; the thread's W[r][j] joins its registers too
+  double vr = 0.0;
+#pragma unroll
+  for (int q = 0; q < kNB; ++q) vr = (q == j) ? vrow[q] : vr;
+  const double w = tau * yy;
+  const double wj = w + alpha * vr;
+  // row c+1 minus panel column j:  V[j][c1] (=1) W[j][r] + W[j][c1] V[j][r]
+  const double x = base - w - wc1 * vr - 2.0 * alpha * vr;
+  const bool live = inr && r >= c1 + 2;
+  if (inr) {
+    ws.W[(int64_t)r * kNB + j] = wj;
     ws.xtmp[r] = x;
     if (r == c1) ws.d[c1] = x;
-    live = r >= c1 + 2;
-    if (live) nrm = x * x;
-    if (r == c1 + 1) al = x;  // the next reflector's alpha (block 0 only)
   }
-  // this block's partials for column c+1, reduced per wave then across waves in order
+#pragma unroll
+  for (int q = 0; q < kNB; ++q) wrow[q] = (q == j) ? wj : wrow[q];
+  // ---- this block's partials for column c+1, summed in a fixed order:
+  // norm, the next alpha (row c+2, block 0), and if column c+1 continues the
+  // panel, V_q . x and W_q . x for q <= j
+  const int jn = ((c1 % kNB) != 0 && c1 + 1 < n) ? j + 1 : 0;
+  const double xl = live ? x : 0.0;
+  double *pwn = ws.pw + (int64_t)(c1 & 1) * ws.maxw * kPW + (int64_t)blockIdx.x * kPW;
   {
-    double v0 = nrm, v1 = al;
+    double v0 = xl * xl, v1 = (inr && r == c1 + 1) ? x : 0.0;
     for (int o = 32; o > 0; o >>= 1) {
       v0 += __shfl_down(v0, o, 64);
       v1 += __shfl_down(v1, o, 64);
     }
     if (lane == 0) {
-      wpart[wv][0] = v0;
-      wpart[wv][1] = v1;
+      red[wv][1] = v0;
+      red[wv][2] = v1;
     }
-    const double xl = live ? x : 0.0;
-    for (int q = 0; q < jn; ++q) {
-      const double vq = live ? ws.V[(int64_t)q * n + r] : 0.0;
-      const double wq = live ? ws.W[(int64_t)q * n + r] : 0.0;
-      double a = vq * xl, b = wq * xl;
-      for (int o = 32; o > 0; o >>= 1) {
-        a += __shfl_down(a, o, 64);
-        b += __shfl_down(b, o, 64);
+  }
+  // components 0..31: V_q . x, 32..63: W_q . x, in chunks of 16 through LDS
+#pragma unroll
+  for (int k0 = 0; k0 < 2 * kNB; k0 += 16) {
+    const int q0 = k0 & (kNB - 1);
+    if (q0 < jn) {  // uniform
+#pragma unroll
+      for (int kk = 0; kk < 16; ++kk) tb[threadIdx.x][kk] = (k0 < kNB ? vrow[q0 + kk] : wrow[q0 + kk]) * xl;
+      __syncthreads();
+      {
+        const int k = threadIdx.x & 15, part = threadIdx.x >> 4;
+        double sum = 0.0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) sum += tb[part * 16 + i][k];
+        tb2[part][k] = sum;
       }
-      if (lane == 0) {
-        wpart[wv][2 + q] = a;
-        wpart[wv][2 + kNB + q] = b;
+      __syncthreads();
+      if (threadIdx.x < 16) {
+        const int q = q0 + threadIdx.x;
+        double sum = 0.0;
+#pragma unroll
+        for (int part = 0; part < 16; ++part) sum += tb2[part][threadIdx.x];
+        pwn[2 + k0 + threadIdx.x] = q < jn ? sum : 0.0;
       }
+      __syncthreads();
+    } else if (threadIdx.x < 16) {
+      pwn[2 + k0 + threadIdx.x] = 0.0;
     }
   }
   __syncthreads();
-  double *pwn = ws.pw + (int64_t)(c1 & 1) * ws.maxw * kPW + (int64_t)blockIdx.x * kPW;
-  for (int t = threadIdx.x; t < kPW; t += 256) {
-    const int q = t < 2 ? 0 : ((t - 2) & (kNB - 1));
-    const bool use = t < 2 || q < jn;
-    pwn[t] = use ? (wpart[0][t] + wpart[1][t]) + (wpart[2][t] + wpart[3][t]) : 0.0;
+  if (threadIdx.x == 0) {
+    pwn[0] = (red[0][1] + red[1][1]) + (red[2][1] + red[3][1]);
+    pwn[1] = (red[0][2] + red[1][2]) + (red[2][2] + red[3][2]);
   }
 }
 
@@ -402,12 +443,12 @@ __global__ __launch_bounds__(256) void k_syr2k(EigWs ws, int32_t n, int32_t k0, 
   const int32_t tr = blockIdx.x / nt, tc = blockIdx.x % nt;
   const int32_t rb = k0 + tr * 64, cb = k0 + tc * 64;
   for (int idx = threadIdx.x; idx < nb * 64; idx += 256) {
-    int q = idx / 64, o = idx % 64;
+    int o = idx / nb, q = idx % nb;  // panel vectors are row-major V[r * kNB + q]
     int32_t r = rb + o, s = cb + o;
-    sv[q][o] = r < n ? ws.V[(int64_t)q * n + r] : 0.0;
-    sw[q][o] = r < n ? ws.W[(int64_t)q * n + r] : 0.0;
-    tv[q][o] = s < n ? ws.V[(int64_t)q * n + s] : 0.0;
-    tw[q][o] = s < n ? ws.W[(int64_t)q * n + s] : 0.0;
+    sv[q][o] = r < n ? ws.V[(int64_t)r * kNB + q] : 0.0;
+    sw[q][o] = r < n ? ws.W[(int64_t)r * kNB + q] : 0.0;
+    tv[q][o] = s < n ? ws.V[(int64_t)s * kNB + q] : 0.0;
+    tw[q][o] = s < n ? ws.W[(int64_t)s * kNB + q] : 0.0;
   }
   __syncthreads();
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
