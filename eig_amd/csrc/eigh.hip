@@ -131,6 +131,24 @@ __device__ inline double block_sum(double v, double *red) {
 //   k_symv(c)   -> ps[b]: sum over its rows of y_r v_r.
 constexpr int kPW = 2 + 2 * kNB;
 
+// sum_{b = start, start+step, ... < end} p[b * stride], in that order, with the
+// loads issued eight at a time so that their latencies overlap
+__device__ inline double sum_strided(const double *__restrict__ p, int64_t stride, int32_t start,
+                                     int32_t step, int32_t end) {
+  double s = 0.0;
+  for (int32_t b = start; b < end; b += 8 * step) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int32_t bb = b + u * step;
+      v[u] = bb < end ? p[(int64_t)bb * stride] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u];
+  }
+  return s;
+}
+
 __device__ inline int32_t wcol_blocks(int32_t n, int32_t c) {  // producer blocks of column c
   return c == 0 ? 1 : (n - c + 255) / 256;
 }
@@ -142,11 +160,7 @@ __device__ inline void col_scalars(const EigWs &ws, int32_t n, int32_t c, double
   const double *pw = ws.pw + (int64_t)(c & 1) * ws.maxw * kPW;
   const int32_t nw = wcol_blocks(n, c);
   const int lane = threadIdx.x & 63;
-  double s0 = 0.0, s1 = 0.0;
-  for (int32_t b = lane; b < nw; b += 64) {
-    s0 += pw[(int64_t)b * kPW];
-    s1 += pw[(int64_t)b * kPW + 1];
-  }
+  double s0 = sum_strided(pw, kPW, lane, 64, nw), s1 = sum_strided(pw + 1, kPW, lane, 64, nw);
   for (int o = 32; o > 0; o >>= 1) {
     s0 += __shfl_down(s0, o, 64);
     s1 += __shfl_down(s1, o, 64);
@@ -189,6 +203,32 @@ __global__ __launch_bounds__(256) void k_symv(EigWs ws, int32_t n, int32_t c, in
   __shared__ double sscale;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int32_t lda = ws.lda;
+  const int32_t r0 = c + 1 + blockIdx.x * kSymvRows;
+  const int32_t a0 = (c + 1) & ~1;
+  const int32_t npair = (((n + 1) & ~1) - a0) >> 1;
+  const double2 *xp = (const double2 *)(ws.xtmp + a0);
+  // rows past n read row r0 instead (in bounds, result discarded), so every load
+  // in the loop is unconditional and all of them are in flight together
+  const double2 *rp[kSymvRows];
+#pragma unroll
+  for (int rr = 0; rr < kSymvRows; ++rr)
+    rp[rr] = (const double2 *)(ws.a + (int64_t)((r0 + rr) < n ? r0 + rr : r0) * lda + a0);
+  double acc[kSymvRows];
+#pragma unroll
+  for (int rr = 0; rr < kSymvRows; ++rr) acc[rr] = 0.0;
+  // the first pairs' loads go out before the reflector scalars are reduced
+  int32_t p = threadIdx.x;
+  double2 x0 = make_double2(0.0, 0.0), x1 = x0, a0v[kSymvRows], a1v[kSymvRows];
+  const bool two = p + 256 < npair;
+  if (two) {
+    x0 = xp[p];
+    x1 = xp[p + 256];
+#pragma unroll
+    for (int rr = 0; rr < kSymvRows; ++rr) {
+      a0v[rr] = rp[rr][p];
+      a1v[rr] = rp[rr][p + 256];
+    }
+  }
   if (wv == 0) {
     double tau, scale, beta;
     col_scalars(ws, n, c, tau, scale, beta);
@@ -203,45 +243,35 @@ __global__ __launch_bounds__(256) void k_symv(EigWs ws, int32_t n, int32_t c, in
   }
   __syncthreads();
   const double scale = sscale;
-  const int32_t r0 = c + 1 + blockIdx.x * kSymvRows;
-  const int32_t a0 = (c + 1) & ~1;
-  const int32_t npair = (((n + 1) & ~1) - a0) >> 1;
-  const double2 *xp = (const double2 *)(ws.xtmp + a0);
-  // rows past n read row r0 instead (in bounds, result discarded), so every load
-  // in the loop is unconditional and all of them are in flight together
-  const double2 *rp[kSymvRows];
-#pragma unroll
-  for (int rr = 0; rr < kSymvRows; ++rr)
-    rp[rr] = (const double2 *)(ws.a + (int64_t)((r0 + rr) < n ? r0 + rr : r0) * lda + a0);
-  double acc[kSymvRows];
-#pragma unroll
-  for (int rr = 0; rr < kSymvRows; ++rr) acc[rr] = 0.0;
   // v: 0 up to column c, 1 at c+1, x*scale beyond (x is zero past n); only the
   // first pair (p = 0) can hold columns <= c+1
-  auto vpair = [&](const double2 xx, int32_t p, double &v0, double &v1) {
+  auto vpair = [&](const double2 xx, int32_t pp, double &v0, double &v1) {
     v0 = xx.x * scale;
     v1 = xx.y * scale;
-    if (p == 0) {
+    if (pp == 0) {
       const int32_t s0 = a0;
       v0 = (s0 <= c) ? 0.0 : ((s0 == c + 1) ? 1.0 : v0);
       v1 = (s0 + 1 == c + 1) ? 1.0 : v1;
     }
   };
-  int32_t p = threadIdx.x;
-  for (; p + 256 < npair; p += 512) {
-    const double2 x0 = xp[p], x1 = xp[p + 256];
-    double2 a0v[kSymvRows], a1v[kSymvRows];
-#pragma unroll
-    for (int rr = 0; rr < kSymvRows; ++rr) {
-      a0v[rr] = rp[rr][p];
-      a1v[rr] = rp[rr][p + 256];
-    }
+  for (bool have = two; have;) {
     double v0, v1, w0, w1;
     vpair(x0, p, v0, v1);
     vpair(x1, p + 256, w0, w1);
 #pragma unroll
     for (int rr = 0; rr < kSymvRows; ++rr)
       acc[rr] += (a0v[rr].x * v0 + a0v[rr].y * v1) + (a1v[rr].x * w0 + a1v[rr].y * w1);
+    p += 512;
+    have = p + 256 < npair;
+    if (have) {
+      x0 = xp[p];
+      x1 = xp[p + 256];
+#pragma unroll
+      for (int rr = 0; rr < kSymvRows; ++rr) {
+        a0v[rr] = rp[rr][p];
+        a1v[rr] = rp[rr][p + 256];
+      }
+    }
   }
   if (p < npair) {
     const double2 x0 = xp[p];
@@ -302,49 +332,9 @@ __global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, in
   const double *pwc = ws.pw + (int64_t)(c & 1) * ws.maxw * kPW;
   const int32_t nw = wcol_blocks(n, c);
   const int32_t gsym = (n - c - 1 + kSymvRows - 1) / kSymvRows;
-  // ---- prologue: every load of the three reductions is issued before any add
-  double yv = 0.0;
-  for (int32_t b = threadIdx.x; b < gsym; b += 256) yv += ws.ps[b];
-  double praw = 0.0;
-  {
-    const int comp = lane;  // 0..31 V dots, 32..63 W dots
-    const int q = comp & (kNB - 1);
-    if (q < j)
-      for (int32_t b = wv; b < nw; b += 4) praw += pwc[(int64_t)b * kPW + 2 + comp];
-  }
-  if (wv == 0) {
-    double tau, scale, beta;
-    col_scalars(ws, n, c, tau, scale, beta);
-    if (lane == 0) {
-      sh[0] = tau;
-      sh[1] = scale;
-    }
-  }
-  for (int o = 32; o > 0; o >>= 1) yv += __shfl_down(yv, o, 64);
-  if (lane == 0) red[wv][0] = yv;
-  pqr[wv][lane] = praw;
-  __syncthreads();
-  const double tau = sh[0], scale = sh[1];
-  const double ydv = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
-  if (threadIdx.x < 2 * kNB) {
-    const int comp = threadIdx.x, q = comp & (kNB - 1);
-    const double raw = (pqr[0][comp] + pqr[1][comp]) + (pqr[2][comp] + pqr[3][comp]);
-    const double at = (comp < kNB) ? ws.V[(int64_t)c1 * kNB + q] : ws.W[(int64_t)c1 * kNB + q];
-    cc[comp] = q < j ? at : 0.0;
-    pq[comp] = q < j ? at + scale * raw : 0.0;  // v = e_{c+1} + scale x beyond c+1
-  }
-  __syncthreads();
-  // alpha = -tau/2 w.v with w.v = tau (y.v - 2 sum_q p_q q_q)
-  double spq = 0.0;
-  for (int q = 0; q < j; ++q) spq += pq[q] * pq[kNB + q];
-  const double alpha = -0.5 * tau * (tau * (ydv - 2.0 * spq));
-  // w_pre at row c+1 (same in every block)
-  double wc1;
-  {
-    double yy = ws.y[c1];
-    for (int q = 0; q < j; ++q) yy -= cc[q] * pq[kNB + q] + cc[kNB + q] * pq[q];
-    wc1 = tau * yy;
-  }
+  // ---- every load that depends on nothing computed here is issued first: this
+  // thread's row (panel coefficients, y, the trailing row c+1), row c+1's panel
+  // coefficients and y, and the three reductions' partials
   const int32_t r = c1 + blockIdx.x * 256 + threadIdx.x;
   const bool inr = r < n;
   const int32_t rs = inr ? r : c1;  // rows past n read row c+1 (in bounds, discarded)
@@ -363,6 +353,51 @@ __global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, in
   }
   double yy = ws.y[rs];
   double base = ws.a[(int64_t)c1 * lda + rs];
+  const double yc1 = ws.y[c1];
+  double atc = 0.0;
+  if (threadIdx.x < 2 * kNB) {
+    const int q = threadIdx.x & (kNB - 1);
+    atc = (threadIdx.x < kNB) ? ws.V[(int64_t)c1 * kNB + q] : ws.W[(int64_t)c1 * kNB + q];
+  }
+  double yv = sum_strided(ws.ps, 1, threadIdx.x, 256, gsym);
+  double praw = 0.0;
+  {
+    const int comp = lane;  // 0..31 V dots, 32..63 W dots
+    const int q = comp & (kNB - 1);
+    if (q < j) praw = sum_strided(pwc + 2 + comp, kPW, wv, 4, nw);
+  }
+  if (wv == 0) {
+    double tau, scale, beta;
+    col_scalars(ws, n, c, tau, scale, beta);
+    if (lane == 0) {
+      sh[0] = tau;
+      sh[1] = scale;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) yv += __shfl_down(yv, o, 64);
+  if (lane == 0) red[wv][0] = yv;
+  pqr[wv][lane] = praw;
+  __syncthreads();
+  const double tau = sh[0], scale = sh[1];
+  const double ydv = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
+  if (threadIdx.x < 2 * kNB) {
+    const int comp = threadIdx.x, q = comp & (kNB - 1);
+    const double raw = (pqr[0][comp] + pqr[1][comp]) + (pqr[2][comp] + pqr[3][comp]);
+    cc[comp] = q < j ? atc : 0.0;
+    pq[comp] = q < j ? atc + scale * raw : 0.0;  // v = e_{c+1} + scale x beyond c+1
+  }
+  __syncthreads();
+  // alpha = -tau/2 w.v with w.v = tau (y.v - 2 sum_q p_q q_q)
+  double spq = 0.0;
+  for (int q = 0; q < j; ++q) spq += pq[q] * pq[kNB + q];
+  const double alpha = -0.5 * tau * (tau * (ydv - 2.0 * spq));
+  // w_pre at row c+1 (same in every block)
+  double wc1;
+  {
+    double y1 = yc1;
+    for (int q = 0; q < j; ++q) y1 -= cc[q] * pq[kNB + q] + cc[kNB + q] * pq[q];
+    wc1 = tau * y1;
+  }
 #pragma unroll
   for (int q = 0; q < kNB; ++q) {
     yy -= vrow[q] * pq[kNB + q] + wrow[q] * pq[q];
@@ -833,7 +868,9 @@ __global__ __launch_bounds__(256) void k_finish_vec(EigWs ws, int32_t n) {
   __shared__ double fac;
   double *x = ws.X + (int64_t)blockIdx.x * n;
   double s = 0.0, best = -1.0;
-  int32_t besti = 0x7fffffff;
+  // start from an element this thread owns (or 0): the index stays in bounds even
+  // when no comparison succeeds (a vector of NaNs)
+  int32_t besti = (int32_t)threadIdx.x < n ? (int32_t)threadIdx.x : 0;
   for (int32_t i = threadIdx.x; i < n; i += 256) {
     s += x[i] * x[i];
     double a = fabs(x[i]);
@@ -859,7 +896,8 @@ __global__ __launch_bounds__(256) void k_finish_vec(EigWs ws, int32_t n) {
   }
   if (threadIdx.x == 0) {
     double f = 1.0 / sqrt(s);
-    if (x[bi[0]] < 0.0) f = -f;
+    const int32_t ib = bi[0];
+    if (ib >= 0 && ib < n && x[ib] < 0.0) f = -f;
     fac = f;
   }
   __syncthreads();
@@ -1027,6 +1065,11 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
     if (std::isnan(asc[i])) return eig_set_err(ctx, EIG_E_CONVERGE, "eigenvalue is NaN");
     if (evals) evals[i] = asc[N - 1 - i];
   }
+  if (nvec > 0 && evecs)
+    for (int64_t i = 0; i < N * nvec; ++i)
+      if (!std::isfinite(evecs[i]))
+        return eig_set_err(ctx, EIG_E_CONVERGE, "eigenvector %lld is not finite",
+                           (long long)(i / N));
   ctx->last_nvec = nvec;
   return EIG_OK;
 }
@@ -1041,6 +1084,8 @@ extern "C" int eig_eigh_xtx(eig_ctx *ctx, int32_t nvec, double *evals, double *e
 extern "C" int eig_eigh(eig_ctx *ctx, const double *a, int32_t n, int32_t nvec, double *evals,
                         double *evecs) {
   if (!ctx || !a || n <= 0) return eig_set_err(ctx, EIG_E_ARG, "eig_eigh: args");
+  for (int64_t i = 0; i < (int64_t)n * n; ++i)
+    if (!std::isfinite(a[i])) return eig_set_err(ctx, EIG_E_ARG, "eig_eigh: matrix is not finite");
   EIG_HIP(ctx, hipSetDevice(ctx->device));
   EIG_TRY(eig_ensure(ctx, ctx->scratch, sizeof(double) * (size_t)n * n));
   EIG_HIP(ctx, hipMemcpyAsync(ctx->scratch.p, a, sizeof(double) * (size_t)n * n,
