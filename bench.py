@@ -237,6 +237,14 @@ def main():
     relerr3, cosmin3, allabs3 = errs(b3)
     flops = 4.0 * N * (N + 1) * M
     ach = flops / (b["syrk_ms"] * 1e-3) / 1e12
+    # HBM bytes per launch of the headline kernel, from the two rocprofv3 --pmc runs
+    # (FETCH_SIZE, WRITE_SIZE) of this workload scaled by a 1 GiB copy
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(tpath) and (N, M) == (5000, 600000):
+        t = json.load(open(tpath)).get("k_syrk_bf16x3<true>")
+        if t:
+            traffic = int(t["read_bytes"] + t["write_bytes"])
     out = {
         "metric": METRIC, "value": round(b["t"], 5), "unit": "s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(b["t"] * 1e3, 3),
@@ -259,7 +267,7 @@ def main():
                       "bf16x3": {"syrk": b3["syrk_ms"], **b3["parts"]},
                       "fp64": {"syrk": f["syrk_ms"], **f["parts"]}},
         "roofline": {"bound": "mfma", "achieved": round(ach, 2), "peak": PEAK_BF16,
-                     "unit": "TFLOP/s", "frac": round(ach / PEAK_BF16, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(ach / PEAK_BF16, 4), "traffic": traffic,
                      "kernel": "k_syrk_bf16x3<true> (4 products)", "algorithmic_flops": flops},
         "fp64_syrk_roofline": {"achieved": round(N * (N + 1) * M / (f["syrk_ms"] * 1e-3) / 1e12, 2),
                                "peak": PEAK_FP64, "unit": "TFLOP/s"},
