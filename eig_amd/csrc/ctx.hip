@@ -253,31 +253,53 @@ extern "C" int eig_synth_packed(eig_ctx *ctx, int64_t nsnp, int32_t nind, int32_
 }
 
 // ---- selection: GENO rows -> tile layout --------------------------------------
-// One block per SNP: the row is staged in LDS, then each thread builds 32-bit
-// words of 16 selected samples (codes re-packed LSB-first, padding = 3).
-__global__ void k_select(const uint8_t *__restrict__ geno, int64_t rlen, int64_t nsnp,
-                         const int32_t *__restrict__ xindex, int32_t nrows, int32_t nst,
-                         uint32_t *__restrict__ sel) {
-  extern __shared__ uint8_t row[];
-  int64_t m = blockIdx.x;
-  const uint8_t *src = geno + m * rlen;
-  for (int64_t b = threadIdx.x; b < rlen; b += blockDim.x) row[b] = src[b];
+// A block re-packs up to 8 consecutive SNPs (fewer when 8 rows exceed 64 KB): their
+// GENO rows are staged in LDS, then wave w
+// builds tiles st = w, w+4, ...; lane (s = lane/8, word = lane%8) forms the 32-bit
+// word of 16 selected samples of SNP m0+s (codes LSB-first, padding = 3), so one
+// wave store writes the 8 SNPs' 32-byte records of a tile, 256 contiguous bytes.
+constexpr int kSelSnps = 8;
+__global__ __launch_bounds__(256) void k_select(const uint8_t *__restrict__ geno, int64_t rlen,
+                                                int64_t nsnp, const int32_t *__restrict__ xindex,
+                                                int32_t nrows, int32_t nst, int32_t nsb,
+                                                int32_t ident, uint32_t *__restrict__ sel) {
+  extern __shared__ uint8_t rows[];  // nsb (<= kSelSnps) x rlen
+  const int64_t m0 = (int64_t)blockIdx.x * nsb;
+  const int ns = (int)min((int64_t)nsb, nsnp - m0);
+  const uint8_t *src = geno + m0 * rlen;
+  const int64_t nbytes = (int64_t)ns * rlen;
+  for (int64_t b = threadIdx.x; b < nbytes; b += blockDim.x) rows[b] = src[b];
   __syncthreads();
-  int32_t nw = nst * kTileWords;
-  for (int32_t w = threadIdx.x; w < nw; w += blockDim.x) {
-    int32_t st = w / kTileWords, ww = w % kTileWords;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int s = lane >> 3, ww = lane & 7;
+  const uint8_t *row = rows + (int64_t)(s < ns ? s : 0) * rlen;
+  for (int32_t st = w; st < nst; st += 4) {
     uint32_t word = 0;
+    if (ident) {  // xindex[j] == j: the word's 16 samples are 4 consecutive GENO bytes
+      const int32_t j0 = st * kTileS + ww * 16;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      int32_t j = st * kTileS + ww * 16 + i;
-      uint32_t code = 3;
-      if (j < nrows) {
-        int32_t k = xindex[j];
-        code = (row[k >> 2] >> ((3 - (k & 3)) << 1)) & 3u;
+      for (int t = 0; t < 4; ++t) {
+        const int32_t jb = j0 + 4 * t;
+        const uint32_t byte = jb < nrows ? row[jb >> 2] : 0xFFu;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const uint32_t code = (jb + i < nrows) ? ((byte >> (6 - 2 * i)) & 3u) : 3u;
+          word |= code << (2 * (4 * t + i));
+        }
       }
-      word |= code << (2 * i);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int32_t j = st * kTileS + ww * 16 + i;
+        uint32_t code = 3;
+        if (j < nrows) {
+          const int32_t k = xindex[j];
+          code = (row[k >> 2] >> ((3 - (k & 3)) << 1)) & 3u;
+        }
+        word |= code << (2 * i);
+      }
     }
-    sel[((int64_t)st * nsnp + m) * kTileWords + ww] = word;
+    if (s < ns) sel[((int64_t)st * nsnp + m0 + s) * kTileWords + ww] = word;
   }
 }
 
@@ -285,9 +307,11 @@ extern "C" int eig_select(eig_ctx *ctx, const int32_t *xindex, int32_t nrows) {
   if (!ctx || !xindex || nrows <= 0) return eig_set_err(ctx, EIG_E_ARG, "eig_select: args");
   if (!ctx->geno.p || ctx->nsnp <= 0) return eig_set_err(ctx, EIG_E_STATE, "no genotypes");
   if (nrows > ctx->nind) return eig_set_err(ctx, EIG_E_ARG, "nrows > nind");
+  int32_t ident = 1;
   for (int32_t i = 0; i < nrows; ++i) {
     if (xindex[i] < 0 || xindex[i] >= ctx->nind || (i > 0 && xindex[i] <= xindex[i - 1]))
       return eig_set_err(ctx, EIG_E_ARG, "xindex must be increasing and in [0,nind)");
+    ident &= xindex[i] == i;
   }
   if (nrows >= 46341) return eig_set_err(ctx, EIG_E_ARG, "size of GRM matrix too large");
   EIG_HIP(ctx, hipSetDevice(ctx->device));
@@ -297,9 +321,12 @@ extern "C" int eig_select(eig_ctx *ctx, const int32_t *xindex, int32_t nrows) {
   EIG_HIP(ctx, hipMemcpyAsync(ctx->xindex.p, xindex, sizeof(int32_t) * (size_t)nrows,
                               hipMemcpyHostToDevice, ctx->stream));
   eig_time_begin(ctx);
-  hipLaunchKernelGGL(k_select, dim3((unsigned)ctx->nsnp), dim3(256), (size_t)ctx->rlen,
-                     ctx->stream, (const uint8_t *)ctx->geno.p, ctx->rlen, ctx->nsnp,
-                     (const int32_t *)ctx->xindex.p, nrows, nst, (uint32_t *)ctx->sel.p);
+  const int32_t nsb = (int32_t)std::min<int64_t>(kSelSnps, 65536 / ctx->rlen);
+  if (nsb < 1) return eig_set_err(ctx, EIG_E_ARG, "eig_select: rows longer than 64 KB");
+  hipLaunchKernelGGL(k_select, dim3((unsigned)((ctx->nsnp + nsb - 1) / nsb)), dim3(256),
+                     (size_t)ctx->rlen * nsb, ctx->stream, (const uint8_t *)ctx->geno.p,
+                     ctx->rlen, ctx->nsnp, (const int32_t *)ctx->xindex.p, nrows, nst, nsb,
+                     ident, (uint32_t *)ctx->sel.p);
   EIG_HIP(ctx, hipGetLastError());
   eig_time_end(ctx, "select");
   ctx->nrows = nrows;

@@ -23,8 +23,9 @@
 //  3. The top nvec eigenvectors of T by inverse iteration (LU with partial
 //     pivoting), Gram-Schmidt inside clusters of close eigenvalues.
 //  4. Back-transform X = Q Y in compact-WY form Q_p = I - V_p T_p V_p^T, last
-//     panel first; all T_p are formed up front in one launch, then one fused
-//     kernel per panel applies the previous panel and reduces V_p^T X.
+//     panel first; all T_p are formed up front in one launch, then per panel one
+//     kernel applies the previous panel and writes per-block dots V_p^T X, and a
+//     one-block kernel sums them in block order into Z_p = T_p V_p^T X.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -33,7 +34,6 @@
 
 constexpr int kNB = 32;      // panel width
 constexpr int kMaxVec = 64;  // max eigenvectors computed
-constexpr int kGroup = 16;   // blocks per first-level reduction group
 
 enum {
   S_GL = 0, S_GU, S_PIVMIN, S_TNORM,
@@ -48,69 +48,16 @@ struct EigWs {
   double *xtmp;   // current reflector column, unscaled (length lda, zero padded)
   double *y;
   double *sc;
-  double *part1, *part2;  // two-level reduction partials (back-transform)
+  double *part1;          // per-block partial dots of the back-transform
   double *pw;             // [2][maxw][kPW] column hand-off partials (by column parity)
   double *ps;             // [maxblk] per-block y.v of k_symv
   int32_t maxw;
-  unsigned *ticket;       // [0] final, [1 + g] group counters
   double *eval;           // ascending
   double *X;              // nvec x n eigenvectors
   double *lu;             // nvec x 5n scratch for inverse iteration
   double *Z;              // kNB x kMaxVec
   double *Tall;           // panels x kNB x kNB
 };
-
-// Deterministic two-level reduction of nv values per block (vals in shared
-// memory).  Returns true in exactly one block, whose `out` (shared, nv; may be
-// the same array as vals) then holds the sums over all blocks in block order.
-// Only wave 0 touches the partial buffers, so the release/acquire of the ticket
-// atomics (issued by its lane 0) covers every cross-block byte; the callers
-// exchange nothing else between blocks inside one kernel.
-__device__ bool hreduce(EigWs &ws, const double *vals, int nv, double *out) {
-  __shared__ bool g_last, f_last;
-  const unsigned b = blockIdx.x, nb = gridDim.x;
-  const unsigned g = b / kGroup, ng = (nb + kGroup - 1) / kGroup;
-  const unsigned gsz = min((unsigned)kGroup, nb - g * kGroup);
-  const bool w0 = threadIdx.x < 64;
-  __syncthreads();
-  if (w0) {
-    for (int t = threadIdx.x; t < nv; t += 64) ws.part1[(int64_t)b * nv + t] = vals[t];
-    if (threadIdx.x == 0) {
-      unsigned prev = __hip_atomic_fetch_add(&ws.ticket[1 + g], 1u, __ATOMIC_ACQ_REL,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-      g_last = (prev == gsz - 1);
-    }
-  }
-  __syncthreads();
-  if (!g_last) return false;
-  if (w0) {
-    __threadfence();
-    for (int t = threadIdx.x; t < nv; t += 64) {
-      double s = 0.0;
-      for (unsigned k = 0; k < gsz; ++k) s += ws.part1[(int64_t)(g * kGroup + k) * nv + t];
-      ws.part2[(int64_t)g * nv + t] = s;
-    }
-    if (threadIdx.x == 0) {
-      ws.ticket[1 + g] = 0;
-      unsigned prev = __hip_atomic_fetch_add(&ws.ticket[0], 1u, __ATOMIC_ACQ_REL,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-      f_last = (prev == ng - 1);
-    }
-  }
-  __syncthreads();
-  if (!f_last) return false;
-  if (w0) {
-    __threadfence();
-    for (int t = threadIdx.x; t < nv; t += 64) {
-      double s = 0.0;
-      for (unsigned k = 0; k < ng; ++k) s += ws.part2[(int64_t)k * nv + t];
-      out[t] = s;
-    }
-    if (threadIdx.x == 0) ws.ticket[0] = 0;
-  }
-  __syncthreads();
-  return true;
-}
 
 // sum over a 256-thread block; result valid in every thread
 __device__ inline double block_sum(double v, double *red) {
@@ -471,51 +418,61 @@ __global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, in
   }
 }
 
-// A[r][s] -= sum_q V[q][r] W[q][s] + W[q][r] V[q][s] for r, s >= k0 (64x64 tiles)
+// A[r][s] -= sum_q V[r][q] W[s][q] + W[r][q] V[s][q] for r, s >= k0: one FP64 MFMA
+// GEMM with K = 2 kNB per 64x64 tile, A operand [V_r | W_r], B operand [W_s | V_s]
+// (4 waves x 32x32, MFMA f64 16x16x4).
+typedef __attribute__((__vector_size__(4 * sizeof(double)))) double eig_f64x4;
+constexpr int kS2K = 2 * kNB;        // GEMM depth
+constexpr int kS2Ld = kS2K + 1;      // padded LDS row (conflict-free 16-row reads)
 __global__ __launch_bounds__(256) void k_syr2k(EigWs ws, int32_t n, int32_t k0, int32_t nb) {
-  __shared__ double sv[kNB][64], sw[kNB][64], tv[kNB][64], tw[kNB][64];
+  __shared__ double sa[64][kS2Ld], sb[64][kS2Ld];
   const int32_t nt = (n - k0 + 63) / 64;
   const int32_t tr = blockIdx.x / nt, tc = blockIdx.x % nt;
   const int32_t rb = k0 + tr * 64, cb = k0 + tc * 64;
-  for (int idx = threadIdx.x; idx < nb * 64; idx += 256) {
-    int o = idx / nb, q = idx % nb;  // panel vectors are row-major V[r * kNB + q]
-    int32_t r = rb + o, s = cb + o;
-    sv[q][o] = r < n ? ws.V[(int64_t)r * kNB + q] : 0.0;
-    sw[q][o] = r < n ? ws.W[(int64_t)r * kNB + q] : 0.0;
-    tv[q][o] = s < n ? ws.V[(int64_t)s * kNB + q] : 0.0;
-    tw[q][o] = s < n ? ws.W[(int64_t)s * kNB + q] : 0.0;
+  // stage: row o of the tile, depth index k (V for k < kNB, W above) -- contiguous rows
+  for (int idx = threadIdx.x; idx < 64 * kS2K; idx += 256) {
+    const int o = idx / kS2K, k = idx % kS2K, q = k & (kNB - 1);
+    const int32_t r = rb + o, c = cb + o;
+    const bool qa = q < nb;
+    const double *src_a = (k < kNB) ? ws.V : ws.W;  // A row r: [V_r | W_r]
+    const double *src_b = (k < kNB) ? ws.W : ws.V;  // B row c: [W_c | V_c]
+    sa[o][k] = (r < n && qa) ? src_a[(int64_t)r * kNB + q] : 0.0;
+    sb[o][k] = (c < n && qa) ? src_b[(int64_t)c * kNB + q] : 0.0;
   }
   __syncthreads();
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  double acc[4][4];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wr = w >> 1, wc = w & 1;
+  eig_f64x4 acc[2][2];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-  for (int q = 0; q < nb; ++q) {
-    double vr[4], wr[4], vs[4], wsv[4];
+    for (int b = 0; b < 2; ++b) acc[a][b] = eig_f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int ks = 0; ks < kS2K / 4; ++ks) {
+    const int k = 4 * ks + (lane >> 4);
+    double av[2], bv[2];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      vr[a] = sv[q][ty + 16 * a];
-      wr[a] = sw[q][ty + 16 * a];
-    }
+    for (int a = 0; a < 2; ++a) av[a] = sa[32 * wr + 16 * a + (lane & 15)][k];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      vs[b] = tv[q][tx + 16 * b];
-      wsv[b] = tw[q][tx + 16 * b];
-    }
+    for (int b = 0; b < 2; ++b) bv[b] = sb[32 * wc + 16 * b + (lane & 15)][k];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-      for (int b = 0; b < 4; ++b) acc[a][b] += vr[a] * wsv[b] + wr[a] * vs[b];
+      for (int b = 0; b < 2; ++b)
+        acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
   }
+  // accumulator layout: lane l, element e -> row l/16 + 4e, column l%16
+  const int32_t lda = ws.lda;
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      int32_t r = rb + ty + 16 * a, s = cb + tx + 16 * b;
-      if (r < n && s < n) ws.a[(int64_t)r * ws.lda + s] -= acc[a][b];
-    }
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int32_t r = rb + 32 * wr + 16 * a + (lane >> 4) + 4 * e;
+        const int32_t c = cb + 32 * wc + 16 * b + (lane & 15);
+        if (r < n && c < n) ws.a[(int64_t)r * lda + c] -= acc[a][b][e];
+      }
 }
 
 // ---------------------------------------------------------------- eigenvalues
@@ -629,6 +586,7 @@ __global__ __launch_bounds__(256) void k_multisect(EigWs ws, int32_t n) {
 }
 
 // ---------------------------------------------------------------- inverse iteration
+constexpr int kInvB = 16;  // recurrence steps per batch of loads
 __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec) {
   __shared__ double red[4];
   __shared__ double dots[2];
@@ -642,10 +600,20 @@ __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec
     double *u0 = ws.lu + (int64_t)kv * 5 * n, *u1 = u0 + n, *u2 = u1 + n, *ml = u2 + n,
            *pv = ml + n;
     double acur = ws.d[0] - lam, bcur = (n > 1) ? ws.e[0] : 0.0;
-#pragma unroll 4
-    for (int32_t i = 0; i < n - 1; ++i) {
-      const double l = ws.e[i];
-      const double an = ws.d[i + 1] - lam, bn = (i + 1 < n - 1) ? ws.e[i + 1] : 0.0;
+    // the recurrence is serial; its operands are loaded kInvB steps at a time so
+    // that one memory latency covers a batch
+    double eb[kInvB + 1], db[kInvB];
+    for (int32_t i0 = 0; i0 < n - 1; i0 += kInvB) {
+#pragma unroll
+      for (int k = 0; k <= kInvB; ++k) eb[k] = (i0 + k < n - 1) ? ws.e[i0 + k] : 0.0;
+#pragma unroll
+      for (int k = 0; k < kInvB; ++k) db[k] = (i0 + k + 1 < n) ? ws.d[i0 + k + 1] : 0.0;
+#pragma unroll
+      for (int k = 0; k < kInvB; ++k) {
+      const int32_t i = i0 + k;
+      if (i < n - 1) {
+      const double l = eb[k];
+      const double an = db[k] - lam, bn = eb[k + 1];
       if (fabs(acur) >= fabs(l)) {
         double a0 = acur;
         if (fabs(a0) < tiny) a0 = copysign(tiny, a0 == 0.0 ? 1.0 : a0);
@@ -658,6 +626,8 @@ __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec
         u0[i] = l; u1[i] = an; u2[i] = bn; ml[i] = m; pv[i] = 1.0;
         acur = bcur - m * an;
         bcur = -m * bn;
+      }
+      }
       }
     }
     if (fabs(acur) < tiny) acur = copysign(tiny, acur == 0.0 ? 1.0 : acur);
@@ -678,24 +648,52 @@ __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec
                    *pv = ml + n;
       double *x = ws.X + (int64_t)kv * n;
       double carry = x[0];
-#pragma unroll 4
-      for (int32_t i = 0; i < n - 1; ++i) {
-        double nx = x[i + 1];
-        if (pv[i] != 0.0) {
-          const double t = carry; carry = nx; nx = t;
+      for (int32_t i0 = 0; i0 < n - 1; i0 += kInvB) {
+        double xb[kInvB], pb[kInvB], mb[kInvB];
+#pragma unroll
+        for (int k = 0; k < kInvB; ++k) {
+          const bool in = i0 + k < n - 1;
+          xb[k] = in ? x[i0 + k + 1] : 0.0;
+          pb[k] = in ? pv[i0 + k] : 0.0;
+          mb[k] = in ? ml[i0 + k] : 0.0;
         }
-        x[i] = carry;
-        carry = nx - ml[i] * carry;
+#pragma unroll
+        for (int k = 0; k < kInvB; ++k) {
+          const int32_t i = i0 + k;
+          if (i < n - 1) {
+            double nx = xb[k];
+            if (pb[k] != 0.0) {
+              const double t = carry; carry = nx; nx = t;
+            }
+            x[i] = carry;
+            carry = nx - mb[k] * carry;
+          }
+        }
       }
       x[n - 1] = carry;
       double x2 = 0.0, x1 = x[n - 1] / u0[n - 1];
       x[n - 1] = x1;
-#pragma unroll 4
-      for (int32_t i = n - 2; i >= 0; --i) {
-        const double xi = (x[i] - u1[i] * x1 - u2[i] * x2) / u0[i];
-        x[i] = xi;
-        x2 = x1;
-        x1 = xi;
+      for (int32_t i0 = n - 2; i0 >= 0; i0 -= kInvB) {
+        double xb[kInvB], ab[kInvB], bb[kInvB], cb[kInvB];
+#pragma unroll
+        for (int k = 0; k < kInvB; ++k) {
+          const int32_t i = i0 - k;
+          const bool in = i >= 0;
+          xb[k] = in ? x[i] : 0.0;
+          ab[k] = in ? u0[i] : 1.0;
+          bb[k] = in ? u1[i] : 0.0;
+          cb[k] = in ? u2[i] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < kInvB; ++k) {
+          const int32_t i = i0 - k;
+          if (i >= 0) {
+            const double xi = (xb[k] - bb[k] * x1 - cb[k] * x2) / ab[k];
+            x[i] = xi;
+            x2 = x1;
+            x1 = xi;
+          }
+        }
       }
     }
     __syncthreads();
@@ -779,16 +777,13 @@ __global__ __launch_bounds__(256) void k_bt_prep(EigWs ws, int32_t n, int32_t la
 }
 
 // Fused step for panel p: X[r] -= sum_q v_{p+1,q}[r] Z[q] (previous panel, if any),
-// then partial dots V_p^T X over this block's rows; the finishing block forms
-// Z <- T_p (V_p^T X) for the next step.
+// then this block's partial dots V_p^T X; k_bt_reduce forms Z <- T_p (V_p^T X).
 __global__ __launch_bounds__(256) void k_bt_step(EigWs ws, int32_t n, int32_t p, int32_t last,
                                                  int32_t nvec, int32_t apply_prev) {
-  // sz (previous panel's Z) is dead once applied, so the dot partials reuse it
   __shared__ double szbuf[kNB * kMaxVec];
   __shared__ double sv[kNB][kBtRows + 1];
   __shared__ double sx[kMaxVec][kBtRows + 1];
   double(*sz)[kMaxVec] = (double(*)[kMaxVec])szbuf;
-  double *mine = szbuf, *tot = szbuf;
   const int32_t lda = ws.lda;
   const int32_t c0 = p * kNB;
   const int32_t nb = min(kNB, last - c0 + 1);
@@ -825,20 +820,35 @@ __global__ __launch_bounds__(256) void k_bt_step(EigWs ws, int32_t n, int32_t p,
     sv[q][i] = (q < nb && i < rows && r > cq) ? ws.a[(int64_t)cq * lda + r] : 0.0;
   }
   __syncthreads();  // sz consumed, sv and sx complete
+  // this block's partial dots V_p^T X over its rows; k_bt_reduce sums them
   const int nd = kNB * nvec;
   for (int idx = threadIdx.x; idx < nd; idx += 256) {
     const int q = idx / nvec, o = idx % nvec;
     double s = 0.0;
     for (int i = 0; i < rows; ++i) s += sv[q][i] * sx[o][i];
-    mine[idx] = s;
+    ws.part1[(int64_t)blockIdx.x * nd + idx] = s;
+  }
+}
+
+// Z_p = T_p (V_p^T X): the partial dots of k_bt_step's nblk blocks summed in block
+// order (three interleaved strands per component, combined in strand order).
+__global__ __launch_bounds__(1024) void k_bt_reduce(EigWs ws, int32_t p, int32_t last,
+                                                    int32_t nvec, int32_t nblk) {
+  __shared__ double acc[3][kNB * kMaxVec];
+  const int nd = kNB * nvec;
+  const int32_t nb = min(kNB, last - p * kNB + 1);
+  for (int t = threadIdx.x; t < 3 * nd; t += 1024) {
+    const int g = t / nd, idx = t % nd;
+    acc[g][idx] = sum_strided(ws.part1 + idx, nd, g, 3, nblk);
   }
   __syncthreads();
-  if (!hreduce(ws, mine, nd, tot)) return;
   const double *T = ws.Tall + (int64_t)p * kNB * kNB;
-  for (int idx = threadIdx.x; idx < nd; idx += 256) {
+  for (int idx = threadIdx.x; idx < nd; idx += 1024) acc[0][idx] += acc[1][idx] + acc[2][idx];
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < nd; idx += 1024) {
     const int q = idx / nvec, o = idx % nvec;
     double s = 0.0;
-    for (int k = q; k < nb; ++k) s += T[q * kNB + k] * tot[k * nvec + o];
+    for (int k = q; k < nb; ++k) s += T[q * kNB + k] * acc[0][k * nvec + o];
     ws.Z[q * kMaxVec + o] = s;
   }
 }
@@ -938,7 +948,6 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
   const int64_t N = n;
   const int32_t lda = (int32_t)(((N + 7) / 8) * 8);
   const int64_t maxblk = (N + kSymvRows - 1) / kSymvRows + 8;
-  const int64_t maxgrp = (maxblk + kGroup - 1) / kGroup + 2;
   const int64_t maxnv = (int64_t)kNB * kMaxVec;
   const int64_t maxw = (N + 255) / 256 + 2;
   const int64_t npanel = (N + kNB - 1) / kNB + 1;
@@ -951,7 +960,7 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
   size_t o_a = add((size_t)N * lda), o_d = add(N), o_e = add(N), o_e2 = add(N),
          o_tau = add(N), o_V = add(kNB * N), o_W = add(kNB * N), o_x = add(lda), o_y = add(N),
          o_sc = add(S_COUNT), o_p1 = add((size_t)maxblk * maxnv),
-         o_p2 = add((size_t)maxgrp * maxnv), o_tk = add((size_t)maxgrp + 8), o_ev = add(N),
+         o_ev = add(N),
          o_X = add((size_t)kMaxVec * N), o_lu = add((size_t)kMaxVec * 5 * N),
          o_Z = add(kNB * kMaxVec), o_T = add((size_t)npanel * kNB * kNB),
          o_pw = add((size_t)2 * maxw * kPW), o_ps = add((size_t)maxblk);
@@ -971,8 +980,6 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
   ws.y = (double *)(base + o_y);
   ws.sc = (double *)(base + o_sc);
   ws.part1 = (double *)(base + o_p1);
-  ws.part2 = (double *)(base + o_p2);
-  ws.ticket = (unsigned *)(base + o_tk);
   ws.eval = (double *)(base + o_ev);
   ws.X = (double *)(base + o_X);
   ws.lu = (double *)(base + o_lu);
@@ -985,7 +992,6 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
   if (lda != n) EIG_HIP(ctx, hipMemsetAsync(ws.a, 0, sizeof(double) * N * lda, st));
   EIG_HIP(ctx, hipMemcpy2DAsync(ws.a, sizeof(double) * lda, d_a, sizeof(double) * N,
                                 sizeof(double) * N, N, hipMemcpyDeviceToDevice, st));
-  EIG_HIP(ctx, hipMemsetAsync(ws.ticket, 0, sizeof(unsigned) * (maxgrp + 8), st));
   EIG_HIP(ctx, hipMemsetAsync(ws.sc, 0, sizeof(double) * S_COUNT, st));
   EIG_HIP(ctx, hipMemsetAsync(ws.e, 0, sizeof(double) * N, st));
   EIG_HIP(ctx, hipMemsetAsync(ws.e2, 0, sizeof(double) * N, st));
@@ -1041,6 +1047,8 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
         const unsigned g = (unsigned)((n - c0 - 1 + kBtRows - 1) / kBtRows);
         hipLaunchKernelGGL(k_bt_step, dim3(g), dim3(256), 0, st, ws, n, p, last, nvec,
                            (int32_t)(p < p_last));
+        hipLaunchKernelGGL(k_bt_reduce, dim3(1), dim3(1024), 0, st, ws, p, last, nvec,
+                           (int32_t)g);
       }
       hipLaunchKernelGGL(k_bt_apply0, dim3((unsigned)((n - 1 + 255) / 256)), dim3(256), 0, st, ws,
                          n, last, nvec);
