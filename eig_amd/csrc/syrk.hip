@@ -342,12 +342,36 @@ int eig_run_syrk(eig_ctx *ctx, int form, int bf16_block, double *trace_scale) {
     if (bf16_block <= 0) bf16_block = 4096;
     if (bf16_block % kKT16 != 0) return eig_set_err(ctx, EIG_E_ARG, "bf16_block % 16 != 0");
   }
-  // split the SNP range so that pairs*splits fills the chip a few times over;
-  // chunks are multiples of the FP32 block (bf16) or of 256 SNPs.
+  // Split the SNP range (split-K).  Every block of one launch runs equally long, so
+  // the launch takes ceil(P*s / slots) rounds of M/s SNPs each, slots being the
+  // blocks the chip holds at once: choose s to minimise that, plus a small charge
+  // per split for the partial tiles it writes and the finaliser reads.  Chunks are
+  // multiples of the FP32 block (bf16) or of 256 SNPs and hold at least 2048 SNPs.
   const int64_t align = split ? bf16_block : 256;
-  int64_t want = std::max<int64_t>(1, (4LL * ctx->num_cu + P - 1) / P);
-  int64_t chunk = (M + want - 1) / want;
-  chunk = ((chunk + align - 1) / align) * align;
+  int occ = 0;
+  if (form == EIG_FORM_FP64)
+    EIG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_syrk_f64, 256, 0));
+  else if (form == EIG_FORM_BF16X3)
+    EIG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_syrk_bf16x3<false>, 256, 0));
+  else
+    EIG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_syrk_bf16x3<true>, 256, 0));
+  const int64_t slots = (int64_t)std::max(occ, 1) * std::max(ctx->num_cu, 1);
+  const int64_t smax = std::max<int64_t>(1, std::min<int64_t>(32, M / std::max<int64_t>(align, 2048)));
+  int64_t chunk = M, best = -1;
+  double best_cost = 0.0;
+  for (int64_t s = 1; s <= smax; ++s) {
+    int64_t c = (M + s - 1) / s;
+    c = ((c + align - 1) / align) * align;
+    const int64_t ns = (M + c - 1) / c;
+    if (ns != s) continue;  // alignment merged two splits: s is not reachable
+    const double rounds = (double)((P * ns + slots - 1) / slots);
+    const double cost = rounds * (double)c / (double)M + 0.001 * (double)ns;
+    if (best < 0 || cost < best_cost - 1e-12) {
+      best = s;
+      best_cost = cost;
+      chunk = c;
+    }
+  }
   int32_t nsplit = (int32_t)((M + chunk - 1) / chunk);
   size_t tile = (size_t)kTileS * kTileS;
   EIG_TRY(eig_ensure(ctx, ctx->part, sizeof(double) * (tile * P * nsplit + 64)));
