@@ -41,6 +41,10 @@ struct eig_ctx {
   DevBuf xindex, sel;
   int32_t nrows = 0, nst = 0;
   bool have_sel = false;
+  // the same codes regrouped per 64-SNP group for the split-BF16 XTX (built on
+  // first use after each selection)
+  DevBuf selg;
+  bool have_selg = false;
 
   // per-SNP statistics (device)
   DevBuf c0, c1, nmiss, status, mean, fancy;

@@ -15,6 +15,7 @@
 //                systematic bias (~3e-6 of the diagonal) that dominates the error
 //                of the top eigenvalues; the fourth product removes it.
 #include <cmath>
+#include <numeric>
 #include <vector>
 
 #include "eig_internal.h"
@@ -137,71 +138,98 @@ __global__ __launch_bounds__(256, 2) void k_syrk_f64(const uint32_t *__restrict_
       }
 }
 
-// ---------------------------------------------------------------- BF16x3 form
-constexpr int kKT16 = 16;      // SNPs per LDS stage (one MFMA k-step)
-constexpr int kRowB = 24;      // bf16 per LDS row: 16 + 8 pad -> 48 B stride
+// ---------------------------------------------------------------- split-BF16 forms
+// Grouped code layout: per sample tile st, 64-SNP group g and sample i, 16 bytes
+// selg[(st * G64 + g) * 128 + i] whose word 2h + (s >> 1), bits 16 (s & 1) + 2 j,
+// hold the code of SNP 64 g + 16 s + 8 h + j (s = 16-SNP stage 0..3, h = half of
+// the stage 0..1, j = 0..7); SNPs past M carry code 3.  A lane of the SYRK that
+// feeds MFMA k-slots 8h..8h+7 of a sample reads one 8-byte word pair per group.
+constexpr int kGrp = 64;
+
+__global__ __launch_bounds__(256) void k_regroup(const uint32_t *__restrict__ sel, int64_t M,
+                                                 int64_t G64, int64_t ngroups,
+                                                 uint4 *__restrict__ selg) {
+  __shared__ uint32_t w[2][kGrp * kTileWords];
+  const int half = threadIdx.x >> 7, i = threadIdx.x & 127;
+  const int64_t gid = (int64_t)blockIdx.x * 2 + half;  // over nst * G64 groups
+  const bool live = gid < ngroups;
+  const int64_t st = live ? gid / G64 : 0, g = live ? gid % G64 : 0;
+  const uint32_t *src = sel + (st * M + g * kGrp) * kTileWords;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = i + 128 * q;  // SNP e >> 3 of the group, word e & 7
+    w[half][e] = (live && g * kGrp + (e >> 3) < M) ? src[e] : 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  if (!live) return;
+  const int wd = i >> 4, sh = 2 * (i & 15);
+  uint32_t o[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const uint32_t c = (w[half][(16 * s + 8 * h + j) * kTileWords + wd] >> sh) & 3u;
+        o[2 * h + (s >> 1)] |= c << (16 * (s & 1) + 2 * j);
+      }
+  selg[gid * kTileS + i] = make_uint4(o[0], o[1], o[2], o[3]);
+}
+
+// Split-BF16 SYRK.  Block = 128x128 tile pair, 4 waves of 64x64; every wave
+// decodes its own MFMA operands straight into registers from a per-group table of
+// level pairs in LDS: for stage s, SNP pair p = 4h + q (SNPs 2p, 2p+1 of the
+// stage) and combo c = code(2p) | code(2p+1) << 2, the table holds the bf16 hi
+// parts of both levels in one word and the lo parts in another -- exactly one
+// dword of the bf16x8 operand.  One table per 64-SNP group, built by the block from
+// the bf16 level entries and double-buffered: one barrier per group.
+//   table word index ((s * 4 + q) * 2 + h) * 16 + c: the two halves h of a wave
+//   read disjoint bank ranges.
+constexpr int kTabWords = 4 * 4 * 2 * 16;
 
 template <bool kLoLo>
-__global__ __launch_bounds__(256, 2) void k_syrk_bf16x3(
-    const uint32_t *__restrict__ sel, int64_t M, const uint32_t *__restrict__ levbf, int32_t P,
-    int64_t chunk, int32_t blk, double *__restrict__ part) {
-  // operand tiles [buf][operand A/B][hi/lo][sample][k]
-  __shared__ __attribute__((aligned(16))) uint16_t zop[2][2][2][kTileS][kRowB];
-  // raw codes [buf][operand][k][word] and levels [buf][k][4]
-  __shared__ uint32_t raw[2][2][kKT16][kTileWords];
-  __shared__ uint32_t lvl[2][kKT16][4];
+__global__ __launch_bounds__(256, 2) void k_syrk_bf16(
+    const uint2 *__restrict__ selg, int64_t G64, int64_t M, const uint32_t *__restrict__ levbf,
+    int32_t P, int64_t chunk, int32_t blk, double *__restrict__ part) {
+  __shared__ uint32_t tab[2][2][kTabWords];  // [buf][hi/lo][entry]
   const int32_t pair = blockIdx.x % P;
   const int32_t split = blockIdx.x / P;
   int32_t ti, tj;
   pair_to_tiles(pair, ti, tj);
-  const bool diag = (ti == tj);
-  const int64_t m_begin = (int64_t)split * chunk;
+  const int64_t m_begin = (int64_t)split * chunk;  // a multiple of kGrp
   const int64_t m_end = min(M, m_begin + chunk);
+  const int32_t ng = (int32_t)((m_end - m_begin + kGrp - 1) / kGrp);
   const int t = threadIdx.x;
   const int lane = t & 63, w = t >> 6;
   const int wr = w >> 1, wc = w & 1;
-  const uint32_t *selA = sel + (int64_t)ti * M * kTileWords;
-  const uint32_t *selB = sel + (int64_t)tj * M * kTileWords;
+  const int h = lane >> 5;
 
-  // fetch: thread t < 128 -> raw word (k = t>>3, word = t&7) of A; 128..255 -> of B;
-  // threads 0..63 also fetch the level entry (k = t>>2, code = t&3).
-  uint32_t rw = 0xFFFFFFFFu, rl = 0;
-  auto fetch = [&](int64_t m0) {
-    const int op = t >> 7, k = (t & 127) >> 3, wd = t & 7;
-    int64_t m = m0 + k;
-    rw = 0xFFFFFFFFu;
-    if (m < m_end) rw = (op ? selB : selA)[m * kTileWords + wd];
-    if (t < 64) {
-      int64_t ml = m0 + (t >> 2);
-      rl = (ml < m_end) ? levbf[ml * 4 + (t & 3)] : 0u;
-    }
-  };
-  auto stash = [&](int buf) {
-    const int op = t >> 7, k = (t & 127) >> 3, wd = t & 7;
-    raw[buf][op][k][wd] = rw;
-    if (t < 64) lvl[buf][t >> 2][t & 3] = rl;
-  };
-  // decode: thread t -> operand op = t>>7 (skipped for B on diagonal tiles),
-  // sample i = t & 127, SNPs 0..15 of the stage -> 16 bf16 hi + 16 lo.
-  auto decode = [&](int rbuf, int obuf) {
-    const int op = t >> 7, i = t & 127;
-    if (op == 1 && diag) return;
-    const int wd = i >> 4, sh = 2 * (i & 15);
-    uint32_t hv[8], lv[8];
+  // codes: rows A0, A1 (tile ti) and B0, B1 (tile tj), half h, as uint2 per group
+  // (row r of the four at src + roff[r]; B's offset from A is wave-uniform)
+  const int64_t g0 = m_begin / kGrp;
+  const uint2 *src = selg + (((int64_t)ti * G64 + g0) * kTileS + 64 * wr + (lane & 31)) * 2 + h;
+  const int64_t offB = ((int64_t)(tj - ti) * G64 * kTileS + 64 * (wc - wr)) * 2;
+  const int64_t roff[4] = {0, 64, offB, offB + 64};
+  // table entries built by this thread: e = t and t + 256
+  //   s = e >> 7, q = (e >> 5) & 3, hh = (e >> 4) & 1, c = e & 15; pair p = 4 hh + q
+  auto lev_fetch = [&](int32_t gi, uint32_t (&lv)[4]) {
 #pragma unroll
-    for (int k2 = 0; k2 < 8; ++k2) {
-      uint32_t c0 = (raw[rbuf][op][2 * k2][wd] >> sh) & 3u;
-      uint32_t c1 = (raw[rbuf][op][2 * k2 + 1][wd] >> sh) & 3u;
-      uint32_t e0 = lvl[rbuf][2 * k2][c0], e1 = lvl[rbuf][2 * k2 + 1][c1];
-      hv[k2] = (e0 & 0xFFFFu) | (e1 << 16);
-      lv[k2] = (e0 >> 16) | (e1 & 0xFFFF0000u);
+    for (int u = 0; u < 2; ++u) {
+      const int e = t + 256 * u;
+      const int s = e >> 7, q = (e >> 5) & 3, hh = (e >> 4) & 1, c = e & 15;
+      const int64_t m0 = m_begin + (int64_t)gi * kGrp + 16 * s + 2 * (4 * hh + q);
+      lv[2 * u] = m0 < m_end ? levbf[m0 * 4 + (c & 3)] : 0u;
+      lv[2 * u + 1] = m0 + 1 < m_end ? levbf[(m0 + 1) * 4 + (c >> 2)] : 0u;
     }
-    uint4 *ph = (uint4 *)&zop[obuf][op][0][i][0];
-    uint4 *pl = (uint4 *)&zop[obuf][op][1][i][0];
-    ph[0] = make_uint4(hv[0], hv[1], hv[2], hv[3]);
-    ph[1] = make_uint4(hv[4], hv[5], hv[6], hv[7]);
-    pl[0] = make_uint4(lv[0], lv[1], lv[2], lv[3]);
-    pl[1] = make_uint4(lv[4], lv[5], lv[6], lv[7]);
+  };
+  auto tab_store = [&](int buf, const uint32_t (&lv)[4]) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int e = t + 256 * u;
+      const uint32_t e0 = lv[2 * u], e1 = lv[2 * u + 1];
+      tab[buf][0][e] = (e0 & 0xFFFFu) | (e1 << 16);
+      tab[buf][1][e] = (e0 >> 16) | (e1 & 0xFFFF0000u);
+    }
   };
 
   f32x16 acc[2][2];
@@ -216,68 +244,89 @@ __global__ __launch_bounds__(256, 2) void k_syrk_bf16x3(
         dacc[a][b][r] = 0.0;
       }
     }
+  auto promote = [&]() {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          dacc[a][b][r] += (double)acc[a][b][r];
+          acc[a][b][r] = 0.f;
+        }
+  };
+  const int32_t kper = blk / 16;  // stages per FP32 block
+  int32_t left = kper;            // stages until the next promotion
 
-  const int64_t nk = (m_end - m_begin + kKT16 - 1) / kKT16;
-  const int64_t kper = blk / kKT16;  // stages per FP32 block
-  // prologue: raw(0) -> LDS, decode(0); raw(1) -> LDS
-  if (nk > 0) {
-    fetch(m_begin);
-    stash(0);
+  uint32_t lv[4];
+  uint2 cur[4], nxt[4];
+  if (ng > 0) {
+    lev_fetch(0, lv);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) cur[r] = src[roff[r]];
+    tab_store(0, lv);
   }
   __syncthreads();
-  if (nk > 0) decode(0, 0);
-  if (nk > 1) {
-    fetch(m_begin + kKT16);
-    stash(1);
-  }
-  __syncthreads();
-  for (int64_t kt = 0; kt < nk; ++kt) {
-    const int ob = (int)(kt & 1);
-    if (kt + 2 < nk) fetch(m_begin + (kt + 2) * kKT16);
-    // compute stage kt
-    {
-      const int opB = diag ? 0 : 1;
-      const int kofs = 8 * (lane >> 5);
-      bf16x8 ah[2], al[2], bh[2], bl[2];
+  const uint32_t hoff = 16u * (uint32_t)h;
+  for (int32_t gi = 0; gi < ng; ++gi) {
+    const int buf = gi & 1;
+    const bool more = gi + 1 < ng;
+    const uint32_t *th = tab[buf][0], *tl = tab[buf][1];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      // the next group's codes and level entries go out once stage 1 has used the
+      // low words (their registers are free then)
+      if (s == 2 && more) {
+        lev_fetch(gi + 1, lv);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) nxt[r] = src[roff[r] + (int64_t)(gi + 1) * (2 * kTileS)];
+      }
+      // operand dwords of row r: q-th pair of this lane's half of stage s
+      auto row = [&](int r, bf16x8 &hi, bf16x8 &lo) {
+        const uint32_t word = (s < 2) ? cur[r].x : cur[r].y;
+        uint32_t hv[4], lw[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t c = (word >> (16 * (s & 1) + 4 * q)) & 15u;
+          const uint32_t idx = (uint32_t)(((s * 4 + q) * 2) * 16) + hoff + c;
+          hv[q] = th[idx];
+          lw[q] = tl[idx];
+        }
+        typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t u32x4;
+        u32x4 hh = {hv[0], hv[1], hv[2], hv[3]}, ll = {lw[0], lw[1], lw[2], lw[3]};
+        hi = __builtin_bit_cast(bf16x8, hh);
+        lo = __builtin_bit_cast(bf16x8, ll);
+      };
+      bf16x8 bh[2], bl[2];
+      row(2, bh[0], bl[0]);
+      row(3, bh[1], bl[1]);
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
-        const int i = 64 * wr + 32 * a + (lane & 31);
-        ah[a] = *(const bf16x8 *)&zop[ob][0][0][i][kofs];
-        al[a] = *(const bf16x8 *)&zop[ob][0][1][i][kofs];
-      }
-#pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        const int j = 64 * wc + 32 * b + (lane & 31);
-        bh[b] = *(const bf16x8 *)&zop[ob][opB][0][j][kofs];
-        bl[b] = *(const bf16x8 *)&zop[ob][opB][1][j][kofs];
-      }
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
+        bf16x8 ah, al;
+        row(a, ah, al);
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
           if (kLoLo)
-            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bl[b], acc[a][b], 0, 0, 0);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bl[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[b], acc[a][b], 0, 0, 0);
         }
+      }
+      // promote the FP32 block sums into FP64 every blk SNPs
+      if (--left == 0) {
+        promote();
+        left = kper;
+      }
     }
-    // promote the FP32 block sums into FP64 every blk SNPs and at the end
-    if (((kt + 1) % kper) == 0 || kt + 1 == nk) {
+    if (more) {
+      tab_store(buf ^ 1, lv);
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-          for (int r = 0; r < 16; ++r) {
-            dacc[a][b][r] += (double)acc[a][b][r];
-            acc[a][b][r] = 0.f;
-          }
+      for (int r = 0; r < 4; ++r) cur[r] = nxt[r];
     }
-    if (kt + 1 < nk) decode(ob ^ 1, ob ^ 1);
-    if (kt + 2 < nk) stash(ob);
     __syncthreads();
   }
+  promote();  // the last, partial block (adds zeros when already promoted)
   double *out = part + ((int64_t)split * P + pair) * (kTileS * kTileS);
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -340,21 +389,21 @@ int eig_run_syrk(eig_ctx *ctx, int form, int bf16_block, double *trace_scale) {
   const bool split = (form == EIG_FORM_BF16X3 || form == EIG_FORM_BF16X4);
   if (split) {
     if (bf16_block <= 0) bf16_block = 4096;
-    if (bf16_block % kKT16 != 0) return eig_set_err(ctx, EIG_E_ARG, "bf16_block % 16 != 0");
+    if (bf16_block % 16 != 0) return eig_set_err(ctx, EIG_E_ARG, "bf16_block % 16 != 0");
   }
   // Split the SNP range (split-K).  Every block of one launch runs equally long, so
   // the launch takes ceil(P*s / slots) rounds of M/s SNPs each, slots being the
   // blocks the chip holds at once: choose s to minimise that, plus a small charge
   // per split for the partial tiles it writes and the finaliser reads.  Chunks are
   // multiples of the FP32 block (bf16) or of 256 SNPs and hold at least 2048 SNPs.
-  const int64_t align = split ? bf16_block : 256;
+  const int64_t align = split ? std::lcm<int64_t>(bf16_block, kGrp) : 256;
   int occ = 0;
   if (form == EIG_FORM_FP64)
     EIG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_syrk_f64, 256, 0));
   else if (form == EIG_FORM_BF16X3)
-    EIG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_syrk_bf16x3<false>, 256, 0));
+    EIG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_syrk_bf16<false>, 256, 0));
   else
-    EIG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_syrk_bf16x3<true>, 256, 0));
+    EIG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_syrk_bf16<true>, 256, 0));
   const int64_t slots = (int64_t)std::max(occ, 1) * std::max(ctx->num_cu, 1);
   const int64_t smax = std::max<int64_t>(1, std::min<int64_t>(32, M / std::max<int64_t>(align, 2048)));
   int64_t chunk = M, best = -1;
@@ -378,18 +427,29 @@ int eig_run_syrk(eig_ctx *ctx, int form, int bf16_block, double *trace_scale) {
   EIG_TRY(eig_ensure(ctx, ctx->xtx, sizeof(double) * (size_t)n * n));
   double *d_tr = (double *)ctx->part.p + tile * P * nsplit;
   unsigned grid = (unsigned)(P * nsplit);
+  const int64_t G64 = (M + kGrp - 1) / kGrp;
+  if (split && !ctx->have_selg) {
+    const int64_t ngroups = (int64_t)T * G64;
+    EIG_TRY(eig_ensure(ctx, ctx->selg, (size_t)ngroups * kTileS * 16));
+    eig_time_begin(ctx);
+    hipLaunchKernelGGL(k_regroup, dim3((unsigned)((ngroups + 1) / 2)), dim3(256), 0, ctx->stream,
+                       (const uint32_t *)ctx->sel.p, M, G64, ngroups, (uint4 *)ctx->selg.p);
+    EIG_HIP(ctx, hipGetLastError());
+    eig_time_end(ctx, "regroup");
+    ctx->have_selg = true;
+  }
   eig_time_begin(ctx);
   if (form == EIG_FORM_FP64) {
     hipLaunchKernelGGL(k_syrk_f64, dim3(grid), dim3(256), 0, ctx->stream,
                        (const uint32_t *)ctx->sel.p, M, (const double *)ctx->lev.p, P, chunk,
                        (double *)ctx->part.p);
   } else if (form == EIG_FORM_BF16X3) {
-    hipLaunchKernelGGL(k_syrk_bf16x3<false>, dim3(grid), dim3(256), 0, ctx->stream,
-                       (const uint32_t *)ctx->sel.p, M, (const uint32_t *)ctx->levbf.p, P,
+    hipLaunchKernelGGL(k_syrk_bf16<false>, dim3(grid), dim3(256), 0, ctx->stream,
+                       (const uint2 *)ctx->selg.p, G64, M, (const uint32_t *)ctx->levbf.p, P,
                        chunk, bf16_block, (double *)ctx->part.p);
   } else {
-    hipLaunchKernelGGL(k_syrk_bf16x3<true>, dim3(grid), dim3(256), 0, ctx->stream,
-                       (const uint32_t *)ctx->sel.p, M, (const uint32_t *)ctx->levbf.p, P,
+    hipLaunchKernelGGL(k_syrk_bf16<true>, dim3(grid), dim3(256), 0, ctx->stream,
+                       (const uint2 *)ctx->selg.p, G64, M, (const uint32_t *)ctx->levbf.p, P,
                        chunk, bf16_block, (double *)ctx->part.p);
   }
   EIG_HIP(ctx, hipGetLastError());
