@@ -187,7 +187,9 @@ __global__ __launch_bounds__(256) void k_regroup(const uint32_t *__restrict__ se
 //   read disjoint bank ranges.
 constexpr int kTabWords = 4 * 4 * 2 * 16;
 
-template <bool kLoLo>
+// kGrpBlk: the FP32 block is a whole number of groups, so promotions happen only at
+// group ends and the four stages of a group form one branch-free block of code.
+template <bool kLoLo, bool kGrpBlk>
 __global__ __launch_bounds__(256, 2) void k_syrk_bf16(
     const uint2 *__restrict__ selg, int64_t G64, int64_t M, const uint32_t *__restrict__ levbf,
     int32_t P, int64_t chunk, int32_t blk, double *__restrict__ part) {
@@ -255,8 +257,8 @@ __global__ __launch_bounds__(256, 2) void k_syrk_bf16(
           acc[a][b][r] = 0.f;
         }
   };
-  const int32_t kper = blk / 16;  // stages per FP32 block
-  int32_t left = kper;            // stages until the next promotion
+  const int32_t kper = kGrpBlk ? blk / kGrp : blk / 16;  // groups / stages per FP32 block
+  int32_t left = kper;  // groups / stages until the next promotion
 
   uint32_t lv[4];
   uint2 cur[4], nxt[4];
@@ -270,16 +272,17 @@ __global__ __launch_bounds__(256, 2) void k_syrk_bf16(
   const uint32_t hoff = 16u * (uint32_t)h;
   for (int32_t gi = 0; gi < ng; ++gi) {
     const int buf = gi & 1;
-    const bool more = gi + 1 < ng;
+    // the next group (the last group loads itself again, unused)
+    const int32_t gn = gi + 1 < ng ? gi + 1 : gi;
     const uint32_t *th = tab[buf][0], *tl = tab[buf][1];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       // the next group's codes and level entries go out once stage 1 has used the
       // low words (their registers are free then)
-      if (s == 2 && more) {
-        lev_fetch(gi + 1, lv);
+      if (s == 2) {
+        lev_fetch(gn, lv);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) nxt[r] = src[roff[r] + (int64_t)(gi + 1) * (2 * kTileS)];
+        for (int r = 0; r < 4; ++r) nxt[r] = src[roff[r] + (int64_t)gn * (2 * kTileS)];
       }
       // operand dwords of row r: q-th pair of this lane's half of stage s
       auto row = [&](int r, bf16x8 &hi, bf16x8 &lo) {
@@ -314,16 +317,18 @@ __global__ __launch_bounds__(256, 2) void k_syrk_bf16(
         }
       }
       // promote the FP32 block sums into FP64 every blk SNPs
-      if (--left == 0) {
+      if (!kGrpBlk && --left == 0) {
         promote();
         left = kper;
       }
     }
-    if (more) {
-      tab_store(buf ^ 1, lv);
-#pragma unroll
-      for (int r = 0; r < 4; ++r) cur[r] = nxt[r];
+    if (kGrpBlk && --left == 0) {
+      promote();
+      left = kper;
     }
+    tab_store(buf ^ 1, lv);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) cur[r] = nxt[r];
     __syncthreads();
   }
   promote();  // the last, partial block (adds zeros when already promoted)
@@ -397,13 +402,19 @@ int eig_run_syrk(eig_ctx *ctx, int form, int bf16_block, double *trace_scale) {
   // per split for the partial tiles it writes and the finaliser reads.  Chunks are
   // multiples of the FP32 block (bf16) or of 256 SNPs and hold at least 2048 SNPs.
   const int64_t align = split ? std::lcm<int64_t>(bf16_block, kGrp) : 256;
+  const bool grpblk = split && bf16_block % kGrp == 0;
+  typedef void (*bf16_kernel)(const uint2 *, int64_t, int64_t, const uint32_t *, int32_t, int64_t,
+                              int32_t, double *);
+  bf16_kernel kb = nullptr;
+  if (form == EIG_FORM_BF16X3)
+    kb = grpblk ? k_syrk_bf16<false, true> : k_syrk_bf16<false, false>;
+  else if (form == EIG_FORM_BF16X4)
+    kb = grpblk ? k_syrk_bf16<true, true> : k_syrk_bf16<true, false>;
   int occ = 0;
   if (form == EIG_FORM_FP64)
     EIG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_syrk_f64, 256, 0));
-  else if (form == EIG_FORM_BF16X3)
-    EIG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_syrk_bf16<false>, 256, 0));
   else
-    EIG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_syrk_bf16<true>, 256, 0));
+    EIG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kb, 256, 0));
   const int64_t slots = (int64_t)std::max(occ, 1) * std::max(ctx->num_cu, 1);
   const int64_t smax = std::max<int64_t>(1, std::min<int64_t>(32, M / std::max<int64_t>(align, 2048)));
   int64_t chunk = M, best = -1;
@@ -443,14 +454,10 @@ int eig_run_syrk(eig_ctx *ctx, int form, int bf16_block, double *trace_scale) {
     hipLaunchKernelGGL(k_syrk_f64, dim3(grid), dim3(256), 0, ctx->stream,
                        (const uint32_t *)ctx->sel.p, M, (const double *)ctx->lev.p, P, chunk,
                        (double *)ctx->part.p);
-  } else if (form == EIG_FORM_BF16X3) {
-    hipLaunchKernelGGL(k_syrk_bf16<false>, dim3(grid), dim3(256), 0, ctx->stream,
-                       (const uint2 *)ctx->selg.p, G64, M, (const uint32_t *)ctx->levbf.p, P,
-                       chunk, bf16_block, (double *)ctx->part.p);
   } else {
-    hipLaunchKernelGGL(k_syrk_bf16<true>, dim3(grid), dim3(256), 0, ctx->stream,
-                       (const uint2 *)ctx->selg.p, G64, M, (const uint32_t *)ctx->levbf.p, P,
-                       chunk, bf16_block, (double *)ctx->part.p);
+    hipLaunchKernelGGL(kb, dim3(grid), dim3(256), 0, ctx->stream, (const uint2 *)ctx->selg.p,
+                       G64, M, (const uint32_t *)ctx->levbf.p, P, chunk, bf16_block,
+                       (double *)ctx->part.p);
   }
   EIG_HIP(ctx, hipGetLastError());
   eig_time_end(ctx, "syrk");

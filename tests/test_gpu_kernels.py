@@ -106,7 +106,7 @@ def test_xtx_fp64_matches_oracle(eng, N, M):
 # 2^-16/sqrt(M) = 1.6e-7 per entry at M=9000.  Long FP32 blocks add a rounding
 # drift that grows with the block length (see the error sweep in DESIGN).
 @pytest.mark.parametrize("form,block,tol", [(1, 4096, 2e-6), (1, 64, 2e-6), (2, 256, 5e-7),
-                                            (2, 64, 5e-7)])
+                                            (2, 64, 5e-7), (2, 48, 5e-7)])
 def test_xtx_split_bf16_error_budget(eng, form, block, tol):
     N, M = 256, 9000
     g = _rand_panel(5, M, N)
