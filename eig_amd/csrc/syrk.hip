@@ -193,7 +193,9 @@ template <bool kLoLo, bool kGrpBlk>
 __global__ __launch_bounds__(256, 2) void k_syrk_bf16(
     const uint2 *__restrict__ selg, int64_t G64, int64_t M, const uint32_t *__restrict__ levbf,
     int32_t P, int64_t chunk, int32_t blk, double *__restrict__ part) {
-  __shared__ uint32_t tab[2][2][kTabWords];  // [buf][hi/lo][entry]
+  // [buf][hi/lo][entry]; the lo table sits 2052 bytes after the hi table, an offset
+  // no paired LDS load can span, so each lookup lands in its operand register
+  __shared__ uint32_t tab[2][2][kTabWords + 1];
   const int32_t pair = blockIdx.x % P;
   const int32_t split = blockIdx.x / P;
   int32_t ti, tj;
@@ -219,9 +221,12 @@ __global__ __launch_bounds__(256, 2) void k_syrk_bf16(
     for (int u = 0; u < 2; ++u) {
       const int e = t + 256 * u;
       const int s = e >> 7, q = (e >> 5) & 3, hh = (e >> 4) & 1, c = e & 15;
+      // SNPs past m_end read SNP m_begin (in bounds) and take 0: no branch
       const int64_t m0 = m_begin + (int64_t)gi * kGrp + 16 * s + 2 * (4 * hh + q);
-      lv[2 * u] = m0 < m_end ? levbf[m0 * 4 + (c & 3)] : 0u;
-      lv[2 * u + 1] = m0 + 1 < m_end ? levbf[(m0 + 1) * 4 + (c >> 2)] : 0u;
+      const uint32_t e0 = levbf[(m0 < m_end ? m0 : m_begin) * 4 + (c & 3)];
+      const uint32_t e1 = levbf[(m0 + 1 < m_end ? m0 + 1 : m_begin) * 4 + (c >> 2)];
+      lv[2 * u] = m0 < m_end ? e0 : 0u;
+      lv[2 * u + 1] = m0 + 1 < m_end ? e1 : 0u;
     }
   };
   auto tab_store = [&](int buf, const uint32_t (&lv)[4]) {
