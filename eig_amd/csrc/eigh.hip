@@ -517,23 +517,19 @@ __global__ void k_gersh(EigWs ws, int32_t n) {
 // reciprocal is v_rcp_f64 refined by two Newton steps (relative error ~1e-16),
 // which perturbs each pivot far below the bisection tolerance.  d and e2 are read
 // four entries ahead of use.
-__device__ inline void sturm2(const double *__restrict__ d, const double *__restrict__ e2,
-                              int32_t n, double x, double x2, double pivmin, int32_t &c1,
-                              int32_t &c2) {
-  int32_t n1 = 0, n2 = 0;
-  double q = d[0] - x, p = d[0] - x2;
-  if (q <= pivmin) { ++n1; q = fmin(q, -pivmin); }
-  if (p <= pivmin) { ++n2; p = fmin(p, -pivmin); }
+// number of eigenvalues of T below x (Sturm count of T - x I), with q clamped away
+// from zero by pivmin as in LAPACK's dstebz
+__device__ inline int32_t sturm1(const double *__restrict__ d, const double *__restrict__ e2,
+                                 int32_t n, double x, double pivmin) {
+  int32_t cnt = 0;
+  double q = d[0] - x;
+  if (q <= pivmin) { ++cnt; q = fmin(q, -pivmin); }
   auto step = [&](double di, double ei) {
-    double r = __builtin_amdgcn_rcp(q), t = __builtin_amdgcn_rcp(p);
+    double r = __builtin_amdgcn_rcp(q);
     r = fma(r, fma(-q, r, 1.0), r);
-    t = fma(t, fma(-p, t, 1.0), t);
     r = fma(r, fma(-q, r, 1.0), r);
-    t = fma(t, fma(-p, t, 1.0), t);
     q = (di - x) - ei * r;
-    p = (di - x2) - ei * t;
-    if (q <= pivmin) { ++n1; q = fmin(q, -pivmin); }
-    if (p <= pivmin) { ++n2; p = fmin(p, -pivmin); }
+    if (q <= pivmin) { ++cnt; q = fmin(q, -pivmin); }
   };
   int32_t i = 1;
   for (; i + 3 < n; i += 4) {
@@ -545,12 +541,13 @@ __device__ inline void sturm2(const double *__restrict__ d, const double *__rest
     step(d3, f3);
   }
   for (; i < n; ++i) step(d[i], e2[i - 1]);
-  c1 = n1;
-  c2 = n2;
+  return cnt;
 }
 
-// 16 lanes per eigenvalue (k-th smallest): each round the 16 lanes evaluate 32
+// 16 lanes per eigenvalue (k-th smallest): each round the 16 lanes evaluate 16
 // interior points of [lo, hi] and keep the sub-interval holding eigenvalue k.
+// (One count per lane: two interleaved counts per lane cut the rounds by a fifth
+// but double each round's work, and the kernel is bound by that work.)
 constexpr int kMS = 16;
 __global__ __launch_bounds__(256) void k_multisect(EigWs ws, int32_t n) {
   const int32_t k = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kMS);
@@ -559,18 +556,22 @@ __global__ __launch_bounds__(256) void k_multisect(EigWs ws, int32_t n) {
   double lo = ws.sc[S_GL], hi = ws.sc[S_GU];
   const double pivmin = ws.sc[S_PIVMIN];
   const double eps = 2.220446049250313e-16;
-  for (int it = 0; it < 64; ++it) {
-    const double tol = 2.0 * eps * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin;
+  // absolute floor eps * ||T||_1, the default of LAPACK's dstebz (ABSTOL <= 0): an
+  // eigenvalue at zero (the centred matrix has one) converges in ~13 rounds instead
+  // of running to the round cap with a relative tolerance alone
+  const double atol = eps * ws.sc[S_TNORM];
+  for (int it = 0; it < 96; ++it) {
+    const double tol = 2.0 * eps * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin + atol;
     const bool done = (hi - lo <= tol);
     // all 16 lanes of a group share lo/hi, so `done` is uniform within a group
     if (__all(done || !active)) break;
-    const double w = (hi - lo) / (double)(2 * kMS + 1);
-    const double x = lo + w * (double)(2 * sub + 1), x2 = lo + w * (double)(2 * sub + 2);
-    int32_t cnt = 0, cnt2 = 0;
-    if (active && !done) sturm2(ws.d, ws.e2, n, x, x2, pivmin, cnt, cnt2);
+    const double w = (hi - lo) / (double)(kMS + 1);
+    const double x = lo + w * (double)(sub + 1);
+    int32_t cnt = 0;
+    if (active && !done) cnt = sturm1(ws.d, ws.e2, n, x, pivmin);
     // new lo = largest point with count <= k ; new hi = smallest point with count > k
-    double cand_lo = (cnt2 <= k) ? x2 : ((cnt <= k) ? x : -1e300);
-    double cand_hi = (cnt > k) ? x : ((cnt2 > k) ? x2 : 1e300);
+    double cand_lo = (cnt <= k) ? x : -1e300;
+    double cand_hi = (cnt > k) ? x : 1e300;
     for (int o = kMS / 2; o > 0; o >>= 1) {
       cand_lo = fmax(cand_lo, __shfl_xor(cand_lo, o, kMS));
       cand_hi = fmin(cand_hi, __shfl_xor(cand_hi, o, kMS));
@@ -586,6 +587,14 @@ __global__ __launch_bounds__(256) void k_multisect(EigWs ws, int32_t n) {
 }
 
 // ---------------------------------------------------------------- inverse iteration
+// 1/a by the hardware reciprocal and two Newton steps (error a few ulp): the LU of
+// T - lambda I keeps pivot reciprocals, so no division sits on a serial chain
+__device__ inline double rcp_nr(double a) {
+  double r = __builtin_amdgcn_rcp(a);
+  r = fma(r, fma(-a, r, 1.0), r);
+  return fma(r, fma(-a, r, 1.0), r);
+}
+
 constexpr int kInvB = 16;  // recurrence steps per batch of loads
 __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec) {
   __shared__ double red[4];
@@ -598,43 +607,59 @@ __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec
     const int kv = tid;
     const double lam = ws.eval[n - 1 - kv];
     double *u0 = ws.lu + (int64_t)kv * 5 * n, *u1 = u0 + n, *u2 = u1 + n, *ml = u2 + n,
-           *pv = ml + n;
+           *pv = ml + n;  // u0 holds pivot reciprocals
     double acur = ws.d[0] - lam, bcur = (n > 1) ? ws.e[0] : 0.0;
-    // the recurrence is serial; its operands are loaded kInvB steps at a time so
-    // that one memory latency covers a batch
-    double eb[kInvB + 1], db[kInvB];
-    for (int32_t i0 = 0; i0 < n - 1; i0 += kInvB) {
+    // the recurrence is serial; its operands are loaded kInvB steps at a time, one
+    // batch ahead of the batch being computed, so memory latency stays hidden
+    auto ldf = [&](int32_t i0, double (&eb)[kInvB + 1], double (&db)[kInvB]) {
 #pragma unroll
       for (int k = 0; k <= kInvB; ++k) eb[k] = (i0 + k < n - 1) ? ws.e[i0 + k] : 0.0;
 #pragma unroll
       for (int k = 0; k < kInvB; ++k) db[k] = (i0 + k + 1 < n) ? ws.d[i0 + k + 1] : 0.0;
+    };
+    auto runf = [&](int32_t i0, const double (&eb)[kInvB + 1], const double (&db)[kInvB]) {
 #pragma unroll
       for (int k = 0; k < kInvB; ++k) {
-      const int32_t i = i0 + k;
-      if (i < n - 1) {
-      const double l = eb[k];
-      const double an = db[k] - lam, bn = eb[k + 1];
-      if (fabs(acur) >= fabs(l)) {
-        double a0 = acur;
-        if (fabs(a0) < tiny) a0 = copysign(tiny, a0 == 0.0 ? 1.0 : a0);
-        const double m = l / a0;
-        u0[i] = a0; u1[i] = bcur; u2[i] = 0.0; ml[i] = m; pv[i] = 0.0;
-        acur = an - m * bcur;
-        bcur = bn;
-      } else {
-        const double m = acur / l;
-        u0[i] = l; u1[i] = an; u2[i] = bn; ml[i] = m; pv[i] = 1.0;
-        acur = bcur - m * an;
-        bcur = -m * bn;
+        const int32_t i = i0 + k;
+        if (i < n - 1) {
+          const double l = eb[k];
+          const double an = db[k] - lam, bn = eb[k + 1];
+          if (fabs(acur) >= fabs(l)) {
+            double a0 = acur;
+            if (fabs(a0) < tiny) a0 = copysign(tiny, a0 == 0.0 ? 1.0 : a0);
+            const double r = rcp_nr(a0);
+            const double m = l * r;
+            u0[i] = r; u1[i] = bcur; u2[i] = 0.0; ml[i] = m; pv[i] = 0.0;
+            acur = an - m * bcur;
+            bcur = bn;
+          } else {
+            const double r = rcp_nr(l);  // l is loaded ahead: off the serial chain
+            const double m = acur * r;
+            u0[i] = r; u1[i] = an; u2[i] = bn; ml[i] = m; pv[i] = 1.0;
+            acur = bcur - m * an;
+            bcur = -m * bn;
+          }
+        }
       }
-      }
+    };
+    {
+      double eA[kInvB + 1], dA[kInvB], eB[kInvB + 1], dB[kInvB];
+      ldf(0, eA, dA);
+      for (int32_t i0 = 0; i0 < n - 1; i0 += 2 * kInvB) {
+        ldf(i0 + kInvB, eB, dB);
+        runf(i0, eA, dA);
+        ldf(i0 + 2 * kInvB, eA, dA);
+        runf(i0 + kInvB, eB, dB);
       }
     }
     if (fabs(acur) < tiny) acur = copysign(tiny, acur == 0.0 ? 1.0 : acur);
-    u0[n - 1] = acur;
+    u0[n - 1] = rcp_nr(acur);
     u1[n - 1] = 0.0; u2[n - 1] = 0.0; ml[n - 1] = 0.0; pv[n - 1] = 0.0;
+  }
+  // start vectors (a fixed hash of index and vector), filled by the whole block
+  for (int kv = 0; kv < nvec; ++kv) {
     double *x = ws.X + (int64_t)kv * n;
-    for (int32_t i = 0; i < n; ++i) {
+    for (int32_t i = tid; i < n; i += 256) {
       uint32_t h = (uint32_t)(i + 1) * 2654435761u ^ (uint32_t)(kv + 7) * 40503u;
       h ^= h >> 15; h *= 2246822519u; h ^= h >> 13;
       x[i] = 0.5 + (double)(h & 0xFFFFFF) / 16777216.0;
@@ -648,8 +673,10 @@ __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec
                    *pv = ml + n;
       double *x = ws.X + (int64_t)kv * n;
       double carry = x[0];
-      for (int32_t i0 = 0; i0 < n - 1; i0 += kInvB) {
-        double xb[kInvB], pb[kInvB], mb[kInvB];
+      // forward: P L; batch i0 reads x[i0+1 .. i0+kInvB] and writes x[i0 .. i0+kInvB-1],
+      // so the next batch's loads (issued before this batch runs) read values no
+      // earlier batch writes
+      auto ldx = [&](int32_t i0, double (&xb)[kInvB], double (&pb)[kInvB], double (&mb)[kInvB]) {
 #pragma unroll
         for (int k = 0; k < kInvB; ++k) {
           const bool in = i0 + k < n - 1;
@@ -657,6 +684,9 @@ __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec
           pb[k] = in ? pv[i0 + k] : 0.0;
           mb[k] = in ? ml[i0 + k] : 0.0;
         }
+      };
+      auto runx = [&](int32_t i0, const double (&xb)[kInvB], const double (&pb)[kInvB],
+                      const double (&mb)[kInvB]) {
 #pragma unroll
         for (int k = 0; k < kInvB; ++k) {
           const int32_t i = i0 + k;
@@ -669,12 +699,23 @@ __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec
             carry = nx - mb[k] * carry;
           }
         }
+      };
+      {
+        double xA[kInvB], pA[kInvB], mA[kInvB], xB[kInvB], pB[kInvB], mB[kInvB];
+        ldx(0, xA, pA, mA);
+        for (int32_t i0 = 0; i0 < n - 1; i0 += 2 * kInvB) {
+          ldx(i0 + kInvB, xB, pB, mB);
+          runx(i0, xA, pA, mA);
+          ldx(i0 + 2 * kInvB, xA, pA, mA);
+          runx(i0 + kInvB, xB, pB, mB);
+        }
       }
       x[n - 1] = carry;
-      double x2 = 0.0, x1 = x[n - 1] / u0[n - 1];
+      // backward: U; batch i0 reads and writes x[i0 - kInvB + 1 .. i0] only
+      double x2 = 0.0, x1 = x[n - 1] * u0[n - 1];
       x[n - 1] = x1;
-      for (int32_t i0 = n - 2; i0 >= 0; i0 -= kInvB) {
-        double xb[kInvB], ab[kInvB], bb[kInvB], cb[kInvB];
+      auto ldb = [&](int32_t i0, double (&xb)[kInvB], double (&ab)[kInvB], double (&bb)[kInvB],
+                     double (&cb)[kInvB]) {
 #pragma unroll
         for (int k = 0; k < kInvB; ++k) {
           const int32_t i = i0 - k;
@@ -684,15 +725,29 @@ __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec
           bb[k] = in ? u1[i] : 0.0;
           cb[k] = in ? u2[i] : 0.0;
         }
+      };
+      auto runb = [&](int32_t i0, const double (&xb)[kInvB], const double (&ab)[kInvB],
+                      const double (&bb)[kInvB], const double (&cb)[kInvB]) {
 #pragma unroll
         for (int k = 0; k < kInvB; ++k) {
           const int32_t i = i0 - k;
           if (i >= 0) {
-            const double xi = (xb[k] - bb[k] * x1 - cb[k] * x2) / ab[k];
+            const double xi = (xb[k] - bb[k] * x1 - cb[k] * x2) * ab[k];
             x[i] = xi;
             x2 = x1;
             x1 = xi;
           }
+        }
+      };
+      {
+        double xA[kInvB], aA[kInvB], bA[kInvB], cA[kInvB];
+        double xB[kInvB], aB[kInvB], bB[kInvB], cB[kInvB];
+        ldb(n - 2, xA, aA, bA, cA);
+        for (int32_t i0 = n - 2; i0 >= 0; i0 -= 2 * kInvB) {
+          ldb(i0 - kInvB, xB, aB, bB, cB);
+          runb(i0, xA, aA, bA, cA);
+          ldb(i0 - 2 * kInvB, xA, aA, bA, cA);
+          runb(i0 - kInvB, xB, aB, bB, cB);
         }
       }
     }
