@@ -424,55 +424,91 @@ __global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, in
 typedef __attribute__((__vector_size__(4 * sizeof(double)))) double eig_f64x4;
 constexpr int kS2K = 2 * kNB;        // GEMM depth
 constexpr int kS2Ld = kS2K + 1;      // padded LDS row (conflict-free 16-row reads)
+// One block per 64-row x kS2Strip-tile strip: the strip's [V|W] rows are staged
+// once and reused for every 64x64 column tile, whose [W|V] rows are staged in turn.
+constexpr int kS2Strip = 4;
 __global__ __launch_bounds__(256) void k_syr2k(EigWs ws, int32_t n, int32_t k0, int32_t nb) {
   __shared__ double sa[64][kS2Ld], sb[64][kS2Ld];
   const int32_t nt = (n - k0 + 63) / 64;
-  const int32_t tr = blockIdx.x / nt, tc = blockIdx.x % nt;
-  const int32_t rb = k0 + tr * 64, cb = k0 + tc * 64;
+  const int32_t ns = (nt + kS2Strip - 1) / kS2Strip;
+  const int32_t tr = blockIdx.x / ns, ts = blockIdx.x % ns;
+  const int32_t rb = k0 + tr * 64;
   // stage: row o of the tile, depth index k (V for k < kNB, W above) -- contiguous rows
   for (int idx = threadIdx.x; idx < 64 * kS2K; idx += 256) {
     const int o = idx / kS2K, k = idx % kS2K, q = k & (kNB - 1);
-    const int32_t r = rb + o, c = cb + o;
-    const bool qa = q < nb;
+    const int32_t r = rb + o;
     const double *src_a = (k < kNB) ? ws.V : ws.W;  // A row r: [V_r | W_r]
-    const double *src_b = (k < kNB) ? ws.W : ws.V;  // B row c: [W_c | V_c]
-    sa[o][k] = (r < n && qa) ? src_a[(int64_t)r * kNB + q] : 0.0;
-    sb[o][k] = (c < n && qa) ? src_b[(int64_t)c * kNB + q] : 0.0;
+    sa[o][k] = (r < n && q < nb) ? src_a[(int64_t)r * kNB + q] : 0.0;
   }
-  __syncthreads();
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int wr = w >> 1, wc = w & 1;
-  eig_f64x4 acc[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = eig_f64x4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-  for (int ks = 0; ks < kS2K / 4; ++ks) {
-    const int k = 4 * ks + (lane >> 4);
-    double av[2], bv[2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a) av[a] = sa[32 * wr + 16 * a + (lane & 15)][k];
-#pragma unroll
-    for (int b = 0; b < 2; ++b) bv[b] = sb[32 * wc + 16 * b + (lane & 15)][k];
+  const int32_t lda = ws.lda;
+  // the output tile's old values (row, column of lane's accumulator elements)
+  auto load_old = [&](int32_t cb, double (&old)[2][2][4]) {
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b)
-        acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int32_t r = rb + 32 * wr + 16 * a + (lane >> 4) + 4 * e;
+          const int32_t c = cb + 32 * wc + 16 * b + (lane & 15);
+          old[a][b][e] = (r < n && c < n) ? ws.a[(int64_t)r * lda + c] : 0.0;
+        }
+  };
+  const int32_t tc_end = min(nt, (ts + 1) * kS2Strip);
+  double old[2][2][4], nold[2][2][4];
+  load_old(k0 + ts * kS2Strip * 64, old);
+  for (int32_t tc = ts * kS2Strip; tc < tc_end; ++tc) {
+    const int32_t cb = k0 + tc * 64;
+    __syncthreads();  // the previous tile's reads of sb are done
+    for (int idx = threadIdx.x; idx < 64 * kS2K; idx += 256) {
+      const int o = idx / kS2K, k = idx % kS2K, q = k & (kNB - 1);
+      const int32_t c = cb + o;
+      const double *src_b = (k < kNB) ? ws.W : ws.V;  // B row c: [W_c | V_c]
+      sb[o][k] = (c < n && q < nb) ? src_b[(int64_t)c * kNB + q] : 0.0;
+    }
+    __syncthreads();
+    // the next tile's old values are in flight while this tile is formed (the last
+    // tile reloads itself, unused)
+    load_old(tc + 1 < tc_end ? cb + 64 : cb, nold);
+    eig_f64x4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[a][b] = eig_f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int ks = 0; ks < kS2K / 4; ++ks) {
+      const int k = 4 * ks + (lane >> 4);
+      double av[2], bv[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) av[a] = sa[32 * wr + 16 * a + (lane & 15)][k];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) bv[b] = sb[32 * wc + 16 * b + (lane & 15)][k];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    // accumulator layout: lane l, element e -> row l/16 + 4e, column l%16
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int32_t r = rb + 32 * wr + 16 * a + (lane >> 4) + 4 * e;
+          const int32_t c = cb + 32 * wc + 16 * b + (lane & 15);
+          if (r < n && c < n) ws.a[(int64_t)r * lda + c] = old[a][b][e] - acc[a][b][e];
+        }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) old[a][b][e] = nold[a][b][e];
   }
-  // accumulator layout: lane l, element e -> row l/16 + 4e, column l%16
-  const int32_t lda = ws.lda;
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int32_t r = rb + 32 * wr + 16 * a + (lane >> 4) + 4 * e;
-        const int32_t c = cb + 32 * wc + 16 * b + (lane & 15);
-        if (r < n && c < n) ws.a[(int64_t)r * lda + c] -= acc[a][b][e];
-      }
 }
 
 // ---------------------------------------------------------------- eigenvalues
@@ -982,8 +1018,8 @@ static void launch_tridiag(EigWs ws, int32_t n, hipStream_t st) {
     // panel end: update the trailing matrix before the next panel's first symv
     if (j == kNB - 1 && c + 2 < n) {
       const int32_t k1 = c + 1;
-      int32_t nt = (n - k1 + 63) / 64;
-      hipLaunchKernelGGL(k_syr2k, dim3((unsigned)(nt * nt)), dim3(256), 0, st, ws, n, k1, kNB);
+      const int32_t nt = (n - k1 + 63) / 64, ns = (nt + kS2Strip - 1) / kS2Strip;
+      hipLaunchKernelGGL(k_syr2k, dim3((unsigned)(nt * ns)), dim3(256), 0, st, ws, n, k1, kNB);
     }
   }
 }
