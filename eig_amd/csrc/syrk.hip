@@ -373,22 +373,43 @@ __global__ void k_trace(const double *__restrict__ part, int32_t P, int32_t nspl
   if (threadIdx.x == 0) tr[0] = red[0];
 }
 
-// full symmetric matrix, scaled by 1/y with y = trace/(nrows-1)
-__global__ void k_unpack_scale(const double *__restrict__ part, int32_t P, int32_t nsplit,
-                               int32_t nrows, const double *__restrict__ tr,
-                               double *__restrict__ xtx) {
-  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t n = nrows;
-  if (idx >= n * n) return;
-  int32_t i = (int32_t)(idx / n), j = (int32_t)(idx % n);
-  int32_t a = i > j ? i : j, b = i > j ? j : i;  // lower-triangle element (a >= b)
-  int32_t ta = a / kTileS, tb = b / kTileS;
-  int32_t p = ta * (ta + 1) / 2 + tb;
-  int64_t off = (int64_t)(a % kTileS) * kTileS + (b % kTileS);
-  double v = 0.0;
-  for (int32_t sp = 0; sp < nsplit; ++sp) v += part[((int64_t)sp * P + p) * (kTileS * kTileS) + off];
-  double y = tr[0] / (double)(nrows - 1);
-  xtx[idx] = v * (1.0 / y);
+// full symmetric matrix, scaled by 1/y with y = trace/(nrows-1): one block per tile
+// pair sums its splits in split order (coalesced tile reads) and writes the tile
+// and, through LDS in 32-row slabs, its transpose.
+__global__ __launch_bounds__(256) void k_unpack_scale(const double *__restrict__ part, int32_t P,
+                                                      int32_t nsplit, int32_t nrows,
+                                                      const double *__restrict__ tr,
+                                                      double *__restrict__ xtx) {
+  __shared__ double sl[32][kTileS + 1];
+  const int32_t p = blockIdx.x;
+  int32_t ti, tj;
+  pair_to_tiles(p, ti, tj);
+  const double inv = 1.0 / (tr[0] / (double)(nrows - 1));
+  const int64_t n = nrows;
+  const int t = threadIdx.x;
+  for (int r0 = 0; r0 < kTileS; r0 += 32) {
+    // slab rows r0..r0+31 of the tile: thread t covers columns t & 127, rows t >> 7 + 2k
+    __syncthreads();
+    for (int k = 0; k < 16; ++k) {
+      const int rr = (t >> 7) + 2 * k, c = t & 127;
+      const int64_t off = (int64_t)(r0 + rr) * kTileS + c;
+      double v = 0.0;
+      for (int32_t sp = 0; sp < nsplit; ++sp) v += part[((int64_t)sp * P + p) * (kTileS * kTileS) + off];
+      v *= inv;
+      sl[rr][c] = v;
+      const int64_t gi = (int64_t)ti * kTileS + r0 + rr, gj = (int64_t)tj * kTileS + c;
+      // lower triangle only: the upper is the transpose of it, so the matrix is
+      // exactly symmetric (a diagonal tile's upper half is not used)
+      if (gi < n && gj <= gi) xtx[gi * n + gj] = v;
+    }
+    __syncthreads();
+    // transpose: row (tile column c) of the slab's 32 values -> xtx[tj*128 + c][ti*128 + r0 ..]
+    for (int k = 0; k < 16; ++k) {
+      const int idx = t + 256 * k, c = idx >> 5, rr = idx & 31;
+      const int64_t gi = (int64_t)tj * kTileS + c, gj = (int64_t)ti * kTileS + r0 + rr;
+      if (gj < n && gi < gj) xtx[gi * n + gj] = sl[rr][c];
+    }
+  }
 }
 
 int eig_run_syrk(eig_ctx *ctx, int form, int bf16_block, double *trace_scale) {
@@ -469,10 +490,9 @@ int eig_run_syrk(eig_ctx *ctx, int form, int bf16_block, double *trace_scale) {
   eig_time_begin(ctx);
   hipLaunchKernelGGL(k_trace, dim3(1), dim3(256), 0, ctx->stream, (const double *)ctx->part.p,
                      P, nsplit, n, d_tr);
-  int64_t nn = (int64_t)n * n;
-  hipLaunchKernelGGL(k_unpack_scale, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0,
-                     ctx->stream, (const double *)ctx->part.p, P, nsplit, n,
-                     (const double *)d_tr, (double *)ctx->xtx.p);
+  hipLaunchKernelGGL(k_unpack_scale, dim3((unsigned)P), dim3(256), 0, ctx->stream,
+                     (const double *)ctx->part.p, P, nsplit, n, (const double *)d_tr,
+                     (double *)ctx->xtx.p);
   EIG_HIP(ctx, hipGetLastError());
   double tr = 0.0;
   EIG_HIP(ctx, hipMemcpyAsync(&tr, d_tr, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
