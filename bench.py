@@ -242,7 +242,7 @@ def main():
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tpath) and (N, M) == (5000, 600000):
-        t = json.load(open(tpath)).get("k_syrk_bf16x3<true>")
+        t = json.load(open(tpath)).get("k_syrk_bf16<true, true>")
         if t:
             traffic = int(t["read_bytes"] + t["write_bytes"])
     out = {
@@ -268,7 +268,7 @@ def main():
                       "fp64": {"syrk": f["syrk_ms"], **f["parts"]}},
         "roofline": {"bound": "mfma", "achieved": round(ach, 2), "peak": PEAK_BF16,
                      "unit": "TFLOP/s", "frac": round(ach / PEAK_BF16, 4), "traffic": traffic,
-                     "kernel": "k_syrk_bf16x3<true> (4 products)", "algorithmic_flops": flops},
+                     "kernel": "k_syrk_bf16<true, true> (4 products)", "algorithmic_flops": flops},
         "fp64_syrk_roofline": {"achieved": round(N * (N + 1) * M / (f["syrk_ms"] * 1e-3) / 1e12, 2),
                                "peak": PEAK_FP64, "unit": "TFLOP/s"},
     }
