@@ -314,7 +314,6 @@ extern "C" int eig_select(eig_ctx *ctx, const int32_t *xindex, int32_t nrows) {
       return eig_set_err(ctx, EIG_E_ARG, "xindex must be increasing and in [0,nind)");
     ident &= xindex[i] == i;
   }
-  if (nrows >= 46341) return eig_set_err(ctx, EIG_E_ARG, "size of GRM matrix too large");
   EIG_HIP(ctx, hipSetDevice(ctx->device));
   int32_t nst = (nrows + kTileS - 1) / kTileS;
   EIG_TRY(eig_ensure(ctx, ctx->xindex, sizeof(int32_t) * (size_t)nrows));

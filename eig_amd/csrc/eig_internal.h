@@ -108,5 +108,8 @@ __device__ __host__ inline float eig_bf16_to_f32(uint16_t h) {
 
 // internal launchers implemented in the kernel translation units
 int eig_run_syrk(eig_ctx *ctx, int form, int bf16_block, double *trace_scale);
+// internal: a helper's own small problem -- the context's published eigenvectors
+// (used by the loadings) and its cached tridiagonal graph are left untouched
 int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, double *evals,
-                 double *evecs);
+                 double *evecs,
+                 bool internal = false);

@@ -1032,10 +1032,13 @@ struct TridiagGraph {
 static TridiagGraph g_graph[16];  // per device
 
 int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, double *evals,
-                 double *evecs) {
+                 double *evecs, bool internal) {
   if (n <= 0) return eig_set_err(ctx, EIG_E_ARG, "eigh: n <= 0");
   if (nvec < 0 || nvec > kMaxVec || nvec > n)
     return eig_set_err(ctx, EIG_E_ARG, "eigh: nvec must be in [0, min(n, %d)]", kMaxVec);
+  // an internal call leaves the caller's stage timing (same events) running
+  auto tbeg = [&]() { if (!internal) eig_time_begin(ctx); };
+  auto tend = [&](const char *what) { if (!internal) eig_time_end(ctx, what); };
   const int64_t N = n;
   const int32_t lda = (int32_t)(((N + 7) / 8) * 8);
   const int64_t maxblk = (N + kSymvRows - 1) / kSymvRows + 8;
@@ -1092,8 +1095,8 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
   EIG_HIP(ctx, hipMemsetAsync(ws.xtmp, 0, sizeof(double) * lda, st));
 
   // ---- tridiagonalisation (graph replay for n above a few hundred)
-  eig_time_begin(ctx);
-  const bool use_graph = n >= 256 && getenv("EIG_NO_GRAPH") == nullptr;
+  tbeg();
+  const bool use_graph = !internal && n >= 256 && getenv("EIG_NO_GRAPH") == nullptr;
   if (use_graph) {
     TridiagGraph &G = g_graph[ctx->device & 15];
     if (G.exec == nullptr || G.n != n || G.a != (void *)ws.a || old != ctx->ework.p) {
@@ -1113,22 +1116,22 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
     launch_tridiag(ws, n, st);
   }
   EIG_HIP(ctx, hipGetLastError());
-  eig_time_end(ctx, "tridiag");
+  tend("tridiag");
   // ---- eigenvalues
-  eig_time_begin(ctx);
+  tbeg();
   hipLaunchKernelGGL(k_gersh, dim3(1), dim3(256), 0, st, ws, n);
   {
     int64_t threads = (int64_t)n * kMS;
     hipLaunchKernelGGL(k_multisect, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, ws, n);
   }
   EIG_HIP(ctx, hipGetLastError());
-  eig_time_end(ctx, "bisect");
+  tend("bisect");
   // ---- eigenvectors
-  eig_time_begin(ctx);
+  tbeg();
   if (nvec > 0) {
     hipLaunchKernelGGL(k_invit, dim3(1), dim3(256), 0, st, ws, n, nvec);
-    eig_time_end(ctx, "invit");
-    eig_time_begin(ctx);
+    tend("invit");
+    tbeg();
     const int32_t last = n - 2;  // last reflector row
     if (last >= 0) {
       const int32_t p_last = last / kNB;
@@ -1147,18 +1150,20 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
     hipLaunchKernelGGL(k_finish_vec, dim3((unsigned)nvec), dim3(256), 0, st, ws, n);
     EIG_HIP(ctx, hipGetLastError());
   } else {
-    eig_time_end(ctx, "invit");
-    eig_time_begin(ctx);
+    tend("invit");
+    tbeg();
   }
-  eig_time_end(ctx, "eigvec");
+  tend("eigvec");
   std::vector<double> asc(N);
   EIG_HIP(ctx, hipMemcpyAsync(asc.data(), ws.eval, sizeof(double) * N, hipMemcpyDeviceToHost, st));
   if (nvec > 0 && evecs)
     EIG_HIP(ctx, hipMemcpyAsync(evecs, ws.X, sizeof(double) * N * nvec, hipMemcpyDeviceToHost, st));
-  EIG_TRY(eig_ensure(ctx, ctx->evecs_dev, sizeof(double) * N * std::max(nvec, 1)));
-  if (nvec > 0)
-    EIG_HIP(ctx, hipMemcpyAsync(ctx->evecs_dev.p, ws.X, sizeof(double) * N * nvec,
-                                hipMemcpyDeviceToDevice, st));
+  if (!internal) {
+    EIG_TRY(eig_ensure(ctx, ctx->evecs_dev, sizeof(double) * N * std::max(nvec, 1)));
+    if (nvec > 0)
+      EIG_HIP(ctx, hipMemcpyAsync(ctx->evecs_dev.p, ws.X, sizeof(double) * N * nvec,
+                                  hipMemcpyDeviceToDevice, st));
+  }
   EIG_HIP(ctx, hipStreamSynchronize(st));
   for (int64_t i = 0; i < N; ++i) {
     if (std::isnan(asc[i])) return eig_set_err(ctx, EIG_E_CONVERGE, "eigenvalue is NaN");
@@ -1169,7 +1174,7 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
       if (!std::isfinite(evecs[i]))
         return eig_set_err(ctx, EIG_E_CONVERGE, "eigenvector %lld is not finite",
                            (long long)(i / N));
-  ctx->last_nvec = nvec;
+  if (!internal) ctx->last_nvec = nvec;
   return EIG_OK;
 }
 

@@ -501,11 +501,170 @@ static int tune_xtb(eig_ctx *ctx, FpcaState &F, const Plan &P, int CT, const dou
 // Eigen-decomposition of the n x n symmetric device matrix G (column-major, ld n):
 // eigenvalues descending in w (host), eigenvectors in the columns of V (device,
 // ld n) in the same order.
+// The same cyclic Jacobi for orders up to kJacLds, with A and V held in LDS (the
+// global-memory version pays an L2 round trip per rotation round).
+constexpr int kJacLds = 48;
+__global__ __launch_bounds__(256) void k_jacobi_lds(double *__restrict__ A, int32_t lda,
+                                                    double *__restrict__ V, int32_t ldv, int32_t n,
+                                                    int32_t max_sweeps, double tol,
+                                                    int32_t *__restrict__ sweeps_out) {
+  __shared__ double sA[kJacLds][kJacLds + 1], sV[kJacLds][kJacLds + 1];
+  __shared__ int32_t idx[kJacLds];
+  __shared__ int32_t pp[kJacLds / 2], pq[kJacLds / 2];
+  __shared__ double cs[kJacLds / 2], sn[kJacLds / 2];
+  __shared__ int rotated;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int np = n + (n & 1);
+  const int half = np / 2;
+  // sA[j][i] = A[j * lda + i] (column j), sV likewise
+  for (int e = tid; e < n * n; e += nt) {
+    const int i = e % n, j = e / n;
+    sA[j][i] = A[(int64_t)j * lda + i];
+    sV[j][i] = (i == j) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  int sweep = 0;
+  for (; sweep < max_sweeps; ++sweep) {
+    if (tid == 0) rotated = 0;
+    for (int e = tid; e < np; e += nt) idx[e] = e;
+    __syncthreads();
+    for (int round = 0; round < np - 1; ++round) {
+      for (int k = tid; k < half; k += nt) {
+        int p = idx[k], q = idx[np - 1 - k];
+        if (p > q) {
+          const int t = p;
+          p = q;
+          q = t;
+        }
+        double c = 1.0, s = 0.0;
+        if (q < n) {
+          const double apq = sA[q][p], app = sA[p][p], aqq = sA[q][q];
+          if (fabs(apq) > tol * sqrt(fabs(app * aqq)) && apq != 0.0) {
+            const double tau = (aqq - app) / (2.0 * apq);
+            const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+            c = 1.0 / sqrt(1.0 + t * t);
+            s = t * c;
+            rotated = 1;
+          }
+        }
+        pp[k] = p;
+        pq[k] = q < n ? q : p;
+        cs[k] = c;
+        sn[k] = q < n ? s : 0.0;
+      }
+      __syncthreads();
+      for (int e = tid; e < n * half; e += nt) {  // columns: A <- A J, V <- V J
+        const int i = e % n, k = e / n;
+        const double c = cs[k], s = sn[k];
+        if (s == 0.0) continue;
+        const int p = pp[k], q = pq[k];
+        const double x = sA[p][i], y = sA[q][i];
+        sA[p][i] = c * x - s * y;
+        sA[q][i] = s * x + c * y;
+        const double u = sV[p][i], v = sV[q][i];
+        sV[p][i] = c * u - s * v;
+        sV[q][i] = s * u + c * v;
+      }
+      __syncthreads();
+      for (int e = tid; e < n * half; e += nt) {  // rows: A <- J^T A
+        const int j = e % n, k = e / n;
+        const double c = cs[k], s = sn[k];
+        if (s == 0.0) continue;
+        const int p = pp[k], q = pq[k];
+        const double x = sA[j][p], y = sA[j][q];
+        sA[j][p] = c * x - s * y;
+        sA[j][q] = s * x + c * y;
+      }
+      __syncthreads();
+      for (int k = tid; k < half; k += nt)
+        if (sn[k] != 0.0) {
+          sA[pq[k]][pp[k]] = 0.0;
+          sA[pp[k]][pq[k]] = 0.0;
+        }
+      {
+        const int e = tid + 1;
+        int v = 0;
+        if (e < np) v = (e == 1) ? idx[np - 1] : idx[e - 1];
+        __syncthreads();
+        if (e < np) idx[e] = v;
+        __syncthreads();
+      }
+    }
+    if (!rotated) break;
+  }
+  for (int e = tid; e < n * n; e += nt) {
+    const int i = e % n, j = e / n;
+    A[(int64_t)j * lda + i] = sA[j][i];
+    V[(int64_t)j * ldv + i] = sV[j][i];
+  }
+  if (tid == 0) sweeps_out[0] = sweep;
+}
+
+// V column k = work column ord[k] (n x n, column-major)
+__global__ void k_gather_cols(double *__restrict__ V, const double *__restrict__ work,
+                              const int32_t *__restrict__ ord, int32_t n) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)n * n) return;
+  const int32_t k = (int32_t)(e / n), i = (int32_t)(e % n);
+  V[e] = work[(int64_t)ord[k] * n + i];
+}
+
+// C (m x n, column-major, ld ldc) = sum over s of part[s] (m x n, ld m), in order
+__global__ void k_sum_parts(const double *__restrict__ part, int32_t S, int32_t m, int32_t n,
+                            double *__restrict__ C, int32_t ldc) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t mn = (int64_t)m * n;
+  if (e >= mn) return;
+  double v = 0.0;
+  for (int32_t sp = 0; sp < S; ++sp) v += part[(int64_t)sp * mn + e];
+  C[(e / m) * ldc + (e % m)] = v;
+}
+
+// C (m x n, column-major, ld ldc) = A B^T, A m x k (ld lda) and B n x k (ld ldb) column-major,
+// for a long k: k is cut into S equal chunks (one strided-batched product) and a
+// remainder, whose partial products are summed in chunk order.  A single product of
+// these shapes gives rocBLAS only a few output tiles to spread a long k over.
+static int gemm_nt_long(eig_ctx *ctx, rocblas_handle h, int32_t m, int32_t n, int64_t k,
+                        const double *A, int64_t lda, const double *B, int64_t ldb, double *C,
+                        int32_t ldc, double *part, size_t part_doubles, hipStream_t st) {
+  const double one = 1.0, zero = 0.0;
+  const int64_t mn = (int64_t)m * n;
+  int64_t S = std::min<int64_t>(64, k / 4096);
+  S = std::min<int64_t>(S, (int64_t)(part_doubles / std::max<int64_t>(mn, 1)) - 1);
+  if (S < 2) {
+    FP_BLAS(ctx, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, m, n,
+                               (rocblas_int)k, &one, A, (rocblas_int)lda, B, (rocblas_int)ldb,
+                               &zero, C, ldc));
+    return EIG_OK;
+  }
+  const int64_t kc = k / S, rem = k - S * kc;
+  FP_BLAS(ctx, rocblas_dgemm_strided_batched(
+                   h, rocblas_operation_none, rocblas_operation_transpose, m, n, (rocblas_int)kc,
+                   &one, A, (rocblas_int)lda, (rocblas_stride)(kc * lda), B, (rocblas_int)ldb,
+                   (rocblas_stride)(kc * ldb), &zero, part, m, (rocblas_stride)mn, (rocblas_int)S));
+  int32_t nparts = (int32_t)S;
+  if (rem > 0) {
+    FP_BLAS(ctx, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, m, n,
+                               (rocblas_int)rem, &one, A + S * kc * lda, (rocblas_int)lda,
+                               B + S * kc * ldb, (rocblas_int)ldb, &zero, part + S * mn, m));
+    ++nparts;
+  }
+  hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, st,
+                     (const double *)part, nparts, m, n, C, ldc);
+  EIG_HIP(ctx, hipGetLastError());
+  return EIG_OK;
+}
+
 static int small_eigh(eig_ctx *ctx, double *G, int32_t n, double *V, std::vector<double> &w,
                       double *work, int32_t *d_sweeps) {
   if (n > kJacMax) return eig_set_err(ctx, EIG_E_ARG, "fpca: basis larger than %d", kJacMax);
   hipStream_t st = ctx->stream;
-  hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(1024), 0, st, G, n, work, n, n, 30, 1e-15, d_sweeps);
+  if (n <= kJacLds)
+    hipLaunchKernelGGL(k_jacobi_lds, dim3(1), dim3(256), 0, st, G, n, work, n, n, 30, 1e-15,
+                       d_sweeps);
+  else
+    hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(1024), 0, st, G, n, work, n, n, 30, 1e-15,
+                       d_sweeps);
   EIG_HIP(ctx, hipGetLastError());
   std::vector<double> a((size_t)n * n);
   int32_t sweeps = 0;
@@ -519,11 +678,13 @@ static int small_eigh(eig_ctx *ctx, double *G, int32_t n, double *V, std::vector
     return a[(size_t)x * n + x] > a[(size_t)y * n + y];
   });
   w.resize(n);
-  for (int32_t k = 0; k < n; ++k) {
-    w[k] = a[(size_t)ord[k] * n + ord[k]];
-    EIG_HIP(ctx, hipMemcpyAsync(V + (size_t)k * n, work + (size_t)ord[k] * n, sizeof(double) * n,
-                                hipMemcpyDeviceToDevice, st));
-  }
+  for (int32_t k = 0; k < n; ++k) w[k] = a[(size_t)ord[k] * n + ord[k]];
+  int32_t *d_ord = d_sweeps + 1;  // kJacMax entries after the sweep count
+  EIG_HIP(ctx, hipMemcpyAsync(d_ord, ord.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_gather_cols, dim3((unsigned)(((int64_t)n * n + 255) / 256)), dim3(256), 0,
+                     st, V, (const double *)work, (const int32_t *)d_ord, n);
+  EIG_HIP(ctx, hipGetLastError());
+  EIG_HIP(ctx, hipStreamSynchronize(st));  // ord lives on this host frame
   return EIG_OK;
 }
 
@@ -576,7 +737,7 @@ extern "C" int eig_fpca(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint64_t 
   const size_t o_g = add(sizeof(double) * kJacMax * kJacMax);
   const size_t o_v = add(sizeof(double) * kJacMax * kJacMax);
   const size_t o_vw = add(sizeof(double) * kJacMax * kJacMax);
-  const size_t o_sm = add(sizeof(double) * (kJacMax + 64));
+  const size_t o_sm = add(sizeof(double) * (kJacMax + 64) + sizeof(int32_t) * (kJacMax + 1));
   const size_t o_cb = add(sizeof(double) * kJacMax * kJacMax);
   const size_t o_ev = add(sizeof(double) * (size_t)K * npad);
   void *old = ctx->scratch2.p;
@@ -588,6 +749,7 @@ extern "C" int eig_fpca(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint64_t 
   double *Vw = (double *)(base + o_vw), *sm = (double *)(base + o_sm), *Ev = (double *)(base + o_ev);
   int32_t *d_sweeps = (int32_t *)(sm + kJacMax);
   double *Cb = (double *)(base + o_cb);
+  const size_t part_doubles = (size_t)Smax * npad * 16 * kMaxCT;
 
   if (!F.blas) {
     FP_BLAS(ctx, rocblas_create_handle(&F.blas));
@@ -666,9 +828,7 @@ extern "C" int eig_fpca(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint64_t 
   // W (nc columns, column-major nc x M at Wp, ld ldq) -= U_r (U_r^T W)
   auto project = [&](double *Wp, int32_t nc) -> int {
     if (r == 0) return EIG_OK;
-    FP_BLAS(ctx, rocblas_dgemm(F.blas, rocblas_operation_none, rocblas_operation_transpose, r, nc,
-                               (rocblas_int)M, &one, U, (rocblas_int)ldq, Wp, (rocblas_int)ldq,
-                               &zero, Cb, r));
+    EIG_TRY(gemm_nt_long(ctx, F.blas, r, nc, M, U, ldq, Wp, ldq, Cb, r, part, part_doubles, st));
     FP_BLAS(ctx, rocblas_dgemm(F.blas, rocblas_operation_transpose, rocblas_operation_none, nc,
                                (rocblas_int)M, r, &mone, Cb, r, U, (rocblas_int)ldq, &one, Wp,
                                (rocblas_int)ldq));
@@ -677,9 +837,7 @@ extern "C" int eig_fpca(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint64_t 
   // Out (keep columns) = orthonormal basis of the nc columns at Wp
   auto orth = [&](const double *Wp, int32_t nc, double *Out, int32_t &keep) -> int {
     keep = 0;
-    FP_BLAS(ctx, rocblas_dgemm(F.blas, rocblas_operation_none, rocblas_operation_transpose, nc, nc,
-                               (rocblas_int)M, &one, Wp, (rocblas_int)ldq, Wp, (rocblas_int)ldq,
-                               &zero, G, nc));
+    EIG_TRY(gemm_nt_long(ctx, F.blas, nc, nc, M, Wp, ldq, Wp, ldq, G, nc, part, part_doubles, st));
     std::vector<double> g((size_t)nc * nc), dsc(nc);
     EIG_HIP(ctx, hipMemcpyAsync(g.data(), G, sizeof(double) * g.size(), hipMemcpyDeviceToHost, st));
     EIG_HIP(ctx, hipStreamSynchronize(st));
@@ -725,9 +883,17 @@ extern "C" int eig_fpca(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint64_t 
   EIG_TRY(tune_xtb(ctx, F, P, CTr, src, ldq, part, Smax, Bm, (int32_t)ldq));
   EIG_TRY(launch_xtb(ctx, P, CTr, F.xtb[CTr].grid, F.xtb[CTr].S, src, ldq, r, part, Bm,
                      (int32_t)ldq, 1.0, st));
-  FP_BLAS(ctx, rocblas_dgemm(F.blas, rocblas_operation_none, rocblas_operation_transpose, r, r,
-                             npad, &one, Bm, (rocblas_int)ldq, Bm, (rocblas_int)ldq, &zero, G, r));
-  EIG_TRY(small_eigh(ctx, G, r, V, w, Vw, d_sweeps));
+  EIG_TRY(gemm_nt_long(ctx, F.blas, r, r, npad, Bm, ldq, Bm, ldq, G, r, part, part_doubles, st));
+  // top-K eigenpairs of B^T B (order r up to 256) by the FP64 eigensolver: all
+  // eigenvalues, K eigenvectors into V (column-major r x K)
+  {
+    std::vector<double> evh((size_t)K * r);
+    w.assign(r, 0.0);
+    EIG_TRY(eig_run_eigh(ctx, G, r, K, w.data(), evh.data(), true));
+    EIG_HIP(ctx, hipMemcpyAsync(V, evh.data(), sizeof(double) * evh.size(), hipMemcpyHostToDevice,
+                                st));
+    EIG_HIP(ctx, hipStreamSynchronize(st));
+  }
   std::vector<double> s(K);
   for (int32_t j = 0; j < K; ++j) {
     if (!(w[j] > 0.0)) return eig_set_err(ctx, EIG_E_NUMERIC, "fpca: singular value is zero");

@@ -511,6 +511,10 @@ extern "C" int eig_xtx(eig_ctx *ctx, int32_t form, int32_t bf16_block, double *t
   if (form != EIG_FORM_FP64 && form != EIG_FORM_BF16X3 && form != EIG_FORM_BF16X4)
     return eig_set_err(ctx, EIG_E_ARG, "unknown XTX form %d", form);
   if (!ctx->have_stats) return eig_set_err(ctx, EIG_E_STATE, "eig_xtx before eig_snpstats");
+  // the N x N matrix is limited as the reference's (smartpca.c:984-988); fastmode is
+  // not, so the selection itself may be larger
+  if (ctx->nrows >= 46341)
+    return eig_set_err(ctx, EIG_E_ARG, "size of GRM matrix too large: nrows: %d", ctx->nrows);
   EIG_HIP(ctx, hipSetDevice(ctx->device));
   ctx->have_xtx = false;
   EIG_TRY(eig_run_syrk(ctx, form, bf16_block, trace_scale));
