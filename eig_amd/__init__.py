@@ -46,6 +46,8 @@ SIGNATURES = {
     "eig_loadings": (C.c_int, [_vp, _f64p, C.c_int32, _vp, _vp, _vp]),
     "eig_lsqproject": (C.c_int, [_vp, _i32p, _f64p, _f64p, _i32p]),
     "eig_fpca": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, _vp, _f64p, _vp]),
+    "eig_fpca_form": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, _vp, C.c_int32,
+                                _f64p, _vp]),
     "eig_dev_alloc": (C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
     "eig_dev_free": (C.c_int, [_vp, _vp]),
     "eig_host_alloc": (C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
@@ -218,12 +220,14 @@ class Engine:
                   "eig_loadings")
         return ff, fx
 
-    def fpca(self, K, L, I, seed=0, omega=None):
-        """fastmode randomized PCA over the selected samples (after snpstats)."""
+    def fpca(self, K, L, I, seed=0, omega=None, form=0):
+        """fastmode randomized PCA over the selected samples (after snpstats);
+        form FORM_FP64 or FORM_BF16X4 (split-BF16 panel products)."""
         om = None if omega is None else np.ascontiguousarray(omega, dtype=np.float64)
         ev = np.zeros(K)
         vec = np.zeros((K, self.nrows))
-        self._chk(self._L.eig_fpca(self.h, K, L, I, seed, _ptr(om), ev, _ptr(vec)), "eig_fpca")
+        self._chk(self._L.eig_fpca_form(self.h, K, L, I, seed, _ptr(om), form, ev, _ptr(vec)),
+                  "eig_fpca_form")
         return ev, vec
 
     def lsqproject(self, use, K):

@@ -108,6 +108,11 @@ __device__ __host__ inline float eig_bf16_to_f32(uint16_t h) {
 
 // internal launchers implemented in the kernel translation units
 int eig_run_syrk(eig_ctx *ctx, int form, int bf16_block, double *trace_scale);
+// builds ctx->selg: per sample tile st, 64-SNP group g and sample i, 16 bytes at
+// [(st * ceil(M/64) + g) * 128 + i]; word 2h + (s >> 1), bits 16 (s & 1) + 2 j hold the
+// code of SNP 16 s + k of the group, h = (k >> 2) & 1, j = (k & 3) | ((k >> 3) << 2)
+constexpr int kGrp = 64;  // SNPs per group of the grouped code layout
+int eig_ensure_selg(eig_ctx *ctx);
 // internal: a helper's own small problem -- the context's published eigenvectors
 // (used by the loadings) and its cached tridiagonal graph are left untouched
 int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, double *evals,

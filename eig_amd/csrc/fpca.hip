@@ -245,17 +245,325 @@ __global__ __launch_bounds__(256) void k_fpca_xtb(const uint32_t *__restrict__ s
   }
 }
 
-// out[i][c] = scale * sum_s part[s][i][c] (splits in order); padding columns of
-// out beyond ldp are left alone.
+// out[i][c] = scale * sum_s part[s][i][c] (splits in order) for c < min(ldp, ldo);
+// padding columns of out beyond ldp are left alone.
 __global__ void k_fpca_reduce(const double *__restrict__ part, int32_t S, int64_t npad,
                               int32_t ldp, double scale, double *__restrict__ out, int32_t ldo) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= npad * ldp) return;
   const int64_t i = idx / ldp;
   const int c = (int)(idx % ldp);
+  if (c >= ldo) return;
   double s = 0.0;
   for (int32_t k = 0; k < S; ++k) s += part[(int64_t)k * npad * ldp + idx];
   out[i * ldo + c] = scale * s;
+}
+
+// ---------------------------------------------------------------- split-BF16 panels
+// The chosen form of the panel products: every operand is split into bf16 hi + lo
+// and the four products hi*hi, hi*lo, lo*hi, lo*lo run on v_mfma_f32_32x32x16_bf16,
+// in FP32 within a block of the reduction and promoted to FP64 across blocks (as
+// the split-BF16 XTX).  The genotype operand is decoded straight into MFMA operand
+// registers from LDS tables of level pairs; the dense panel is split once per
+// product into transposed bf16 hi / lo arrays.
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float fp_f32x16;
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 fp_bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t fp_u32x4;
+
+// fastmode levels lf as bf16 hi (low half) | lo (high half) per (SNP, code)
+__global__ void k_fpca_levbf(const double *__restrict__ lf, int64_t M, uint32_t *__restrict__ lb) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 4 * M) return;
+  const double v = lf[e];
+  const uint16_t hi = eig_bf16_rne((float)v);
+  const uint16_t lo = eig_bf16_rne((float)(v - (double)eig_bf16_to_f32(hi)));
+  lb[e] = (uint32_t)hi | ((uint32_t)lo << 16);
+}
+
+// th/tl[c][r] = bf16 hi / lo of src[r * lds + c] for r < rows, c < C; zero up to
+// rpad rows and cpad columns (row stride of th/tl: rpad)
+__global__ void k_fpca_split_t(const double *__restrict__ src, int64_t rows, int64_t lds,
+                               int32_t C, int64_t rpad, int32_t cpad, uint16_t *__restrict__ th,
+                               uint16_t *__restrict__ tl) {
+  __shared__ double tile[32][33];
+  const int64_t r0 = (int64_t)blockIdx.x * 32;
+  const int32_t c0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 8 rows at a time
+  for (int k = ty; k < 32; k += 8) {
+    const int64_t r = r0 + k;
+    const int32_t c = c0 + tx;
+    tile[k][tx] = (r < rows && c < C) ? src[r * lds + c] : 0.0;
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int32_t c = c0 + k;
+    const int64_t r = r0 + tx;
+    if (c >= cpad || r >= rpad) continue;
+    const double v = tile[tx][k];
+    const uint16_t hi = eig_bf16_rne((float)v);
+    const uint16_t lo = eig_bf16_rne((float)(v - (double)eig_bf16_to_f32(hi)));
+    th[(int64_t)c * rpad + r] = hi;
+    tl[(int64_t)c * rpad + r] = lo;
+  }
+}
+
+// Y = X A.  Persistent over SNP tiles of 128 (4 waves x 32 SNPs); a wave's 32 SNPs x
+// 32CT columns accumulate in FP32 over blkt sample tiles, then in FP64.  The table of
+// SNP s (row s, stride 17 against bank conflicts) maps the codes c = code(2p) |
+// code(2p+1) << 2 of two consecutive samples to their two bf16 hi (table 0) or lo
+// (table 1) parts -- one dword of the operand.  Ah/Al: [column][sample], row stride ldt.
+template <int CT>
+__global__ __launch_bounds__(256, 2) void k_fpca_xa_bf16(
+    const uint32_t *__restrict__ sel, int64_t M, int32_t nst, const uint32_t *__restrict__ lbf,
+    const uint16_t *__restrict__ Ah, const uint16_t *__restrict__ Al, int64_t ldt,
+    double *__restrict__ Y, int64_t ldy, int32_t C, int32_t blkt) {
+  // two tables 8708 bytes apart: no paired LDS load spans that, so every lookup
+  // lands in its operand register
+  __shared__ uint32_t tt[2][128 * 17 + 1];
+  __shared__ __attribute__((aligned(16))) uint16_t sA[2][32 * CT][128 + 8];
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63, h = lane >> 5, r = lane & 31;
+  const int64_t ntiles = (M + 127) / 128;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t mb = tile * 128;
+    __syncthreads();  // the previous tile's table reads are done
+    for (int e = t; e < 128 * 16; e += 256) {
+      const int sl = e >> 4, c = e & 15;
+      const int64_t m = mb + sl;
+      uint32_t e0 = 0u, e1 = 0u;
+      if (m < M) {
+        e0 = lbf[4 * m + (c & 3)];
+        e1 = lbf[4 * m + (c >> 2)];
+      }
+      tt[0][sl * 17 + c] = (e0 & 0xFFFFu) | (e1 << 16);
+      tt[1][sl * 17 + c] = (e0 >> 16) | (e1 & 0xFFFF0000u);
+    }
+    const int64_t m = mb + 32 * w + r;  // this lane's SNP
+    const uint32_t rowoff = (uint32_t)((32 * w + r) * 17);
+    fp_f32x16 acc[CT];
+    double dacc[CT][16];
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        acc[ct][q] = 0.f;
+        dacc[ct][q] = 0.0;
+      }
+    auto promote = [&]() {
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          dacc[ct][q] += (double)acc[ct][q];
+          acc[ct][q] = 0.f;
+        }
+    };
+    int32_t left = blkt;
+    for (int32_t st = 0; st < nst; ++st) {
+      __syncthreads();  // the previous sample tile's panel reads are done
+      for (int q = t; q < 2 * 32 * CT * 16; q += 256) {
+        const int hl = q / (32 * CT * 16), rem = q % (32 * CT * 16), c = rem >> 4, ch = rem & 15;
+        const uint16_t *src = (hl ? Al : Ah) + (int64_t)c * ldt + (int64_t)st * kTileS + ch * 8;
+        *(uint4 *)&sA[hl][c][ch * 8] = *(const uint4 *)src;
+      }
+      uint32_t wd[8];
+      if (m < M) {
+        const uint4 a = *(const uint4 *)(sel + ((int64_t)st * M + m) * kTileWords);
+        const uint4 b = *(const uint4 *)(sel + ((int64_t)st * M + m) * kTileWords + 4);
+        wd[0] = a.x; wd[1] = a.y; wd[2] = a.z; wd[3] = a.w;
+        wd[4] = b.x; wd[5] = b.y; wd[6] = b.z; wd[7] = b.w;
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) wd[j] = 0xFFFFFFFFu;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {  // 16-sample steps of the tile
+        const uint32_t bits = wd[j] >> (16 * h);
+        fp_u32x4 xh, xl;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t c = (bits >> (4 * q)) & 15u;
+          xh[q] = tt[0][rowoff + c];
+          xl[q] = tt[1][rowoff + c];
+        }
+        const fp_bf16x8 ah = __builtin_bit_cast(fp_bf16x8, xh), al = __builtin_bit_cast(fp_bf16x8, xl);
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+          const fp_bf16x8 bh = *(const fp_bf16x8 *)&sA[0][32 * ct + r][16 * j + 8 * h];
+          const fp_bf16x8 bl = *(const fp_bf16x8 *)&sA[1][32 * ct + r][16 * j + 8 * h];
+          acc[ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bl, acc[ct], 0, 0, 0);
+          acc[ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[ct], 0, 0, 0);
+          acc[ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[ct], 0, 0, 0);
+          acc[ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[ct], 0, 0, 0);
+        }
+      }
+      if (--left == 0) {
+        promote();
+        left = blkt;
+      }
+    }
+    promote();
+    // accumulator: lane l, element q -> row 8(q>>2) + 4(l>>5) + (q&3), column l&31
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int64_t row = mb + 32 * w + 8 * (q >> 2) + 4 * h + (q & 3);
+        const int col = 32 * ct + r;
+        if (row < M && col < C) Y[row * ldy + col] = dacc[ct][q];
+      }
+  }
+}
+
+// Partial X^T B.  Items (sample tile, SNP split, 64-column group); a block covers one
+// sample tile (4 waves x 32 samples) x 64 columns over SNPs [s*mchunk, ...) (mchunk a
+// multiple of 64).  The genotype operand comes from the grouped layout through
+// per-group tables of level pairs (the split-BF16 XTX's scheme); the panel operand
+// from Bh/Bl [column][SNP] (row stride ldt >= 64 * ceil(M/64)), staged per group.
+// part[s][i][c] with row stride ldp.
+constexpr int kFpTab = 4 * 4 * 2 * 16;
+__global__ __launch_bounds__(256, 2) void k_fpca_xtb_bf16(
+    const uint2 *__restrict__ selg, int64_t G64, int64_t M, int32_t nst,
+    const uint32_t *__restrict__ lbf, const uint16_t *__restrict__ Bh,
+    const uint16_t *__restrict__ Bl, int64_t ldt, int32_t ncg, int32_t S, int64_t mchunk,
+    int32_t blkg, double *__restrict__ part, int32_t ldp) {
+  __shared__ uint32_t tab[2][2][kFpTab + 1];  // [buf][hi/lo][entry], as the XTX kernel
+  __shared__ __attribute__((aligned(16))) uint16_t sB[2][2][64][64 + 8];  // [buf][hi/lo][col][snp]
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63, h = lane >> 5, r = lane & 31;
+  const int64_t items = (int64_t)nst * S * ncg;
+  const int64_t npad = (int64_t)nst * kTileS;
+  for (int64_t it = blockIdx.x; it < items; it += gridDim.x) {
+    const int32_t st = (int32_t)(it % nst);
+    const int64_t rest = it / nst;
+    const int32_t s = (int32_t)(rest % S), cg = (int32_t)(rest / S);
+    const int64_t m_begin = (int64_t)s * mchunk;
+    const int64_t m_end = min(M, m_begin + mchunk);
+    if (m_begin >= m_end) {  // an empty split (M not filling S chunks): zero partial
+      double *out = part + ((int64_t)s * npad + (int64_t)st * kTileS) * ldp;
+      for (int e = t; e < kTileS * 64; e += 256) out[(int64_t)(e >> 6) * ldp + 64 * cg + (e & 63)] = 0.0;
+      continue;
+    }
+    const int32_t ng = (int32_t)((m_end - m_begin + kGrp - 1) / kGrp);
+    const int64_t g0 = m_begin / kGrp;
+    const uint2 *src = selg + (((int64_t)st * G64 + g0) * kTileS + 32 * w + r) * 2 + h;
+    auto lev_fetch = [&](int32_t gi, uint32_t (&lv)[4]) {
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int e = t + 256 * u;
+        const int s4 = e >> 7, q = (e >> 5) & 3, hh = (e >> 4) & 1, c = e & 15;
+        const int64_t m0 = m_begin + (int64_t)gi * kGrp + 16 * s4 + 2 * (4 * hh + q);
+        const uint32_t e0 = lbf[(m0 < m_end ? m0 : m_begin) * 4 + (c & 3)];
+        const uint32_t e1 = lbf[(m0 + 1 < m_end ? m0 + 1 : m_begin) * 4 + (c >> 2)];
+        lv[2 * u] = m0 < m_end ? e0 : 0u;
+        lv[2 * u + 1] = m0 + 1 < m_end ? e1 : 0u;
+      }
+    };
+    auto tab_store = [&](int buf, const uint32_t (&lv)[4]) {
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int e = t + 256 * u;
+        const uint32_t e0 = lv[2 * u], e1 = lv[2 * u + 1];
+        tab[buf][0][e] = (e0 & 0xFFFFu) | (e1 << 16);
+        tab[buf][1][e] = (e0 >> 16) | (e1 & 0xFFFF0000u);
+      }
+    };
+    // panel chunk of group gi: 2 x 64 columns x 64 SNPs = 1024 uint4, four per thread
+    // (u = 0, 1: hi rows t >> 3 and 32 + (t >> 3); u = 2, 3: the same lo rows)
+    const int pc = t >> 3, pch = t & 7;
+    const uint16_t *ph0 = Bh + (int64_t)(64 * cg + pc) * ldt + m_begin + pch * 8;
+    const uint16_t *pl0 = Bl + (int64_t)(64 * cg + pc) * ldt + m_begin + pch * 8;
+    const int64_t p32 = 32 * ldt;
+    uint4 pv0, pv1, pv2, pv3;
+    auto pan_fetch = [&](int32_t gi) {
+      const int64_t o = (int64_t)gi * kGrp;
+      pv0 = *(const uint4 *)(ph0 + o);
+      pv1 = *(const uint4 *)(ph0 + p32 + o);
+      pv2 = *(const uint4 *)(pl0 + o);
+      pv3 = *(const uint4 *)(pl0 + p32 + o);
+    };
+    auto pan_store = [&](int buf) {
+      *(uint4 *)&sB[buf][0][pc][pch * 8] = pv0;
+      *(uint4 *)&sB[buf][0][32 + pc][pch * 8] = pv1;
+      *(uint4 *)&sB[buf][1][pc][pch * 8] = pv2;
+      *(uint4 *)&sB[buf][1][32 + pc][pch * 8] = pv3;
+    };
+    fp_f32x16 acc[2];
+    double dacc[2][16];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        acc[ct][q] = 0.f;
+        dacc[ct][q] = 0.0;
+      }
+    auto promote = [&]() {
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          dacc[ct][q] += (double)acc[ct][q];
+          acc[ct][q] = 0.f;
+        }
+    };
+    uint32_t lv[4];
+    uint2 cur = src[0], nxt = cur;
+    __syncthreads();  // the previous item's reads of tab and sB are done
+    lev_fetch(0, lv);
+    pan_fetch(0);
+    tab_store(0, lv);
+    pan_store(0);
+    __syncthreads();
+    int32_t left = blkg;
+    for (int32_t gi = 0; gi < ng; ++gi) {
+      const int buf = gi & 1;
+      const int32_t gn = gi + 1 < ng ? gi + 1 : gi;  // the last group loads itself again
+      const uint32_t *th = tab[buf][0], *tl = tab[buf][1];
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) {
+        if (s4 == 2) {
+          lev_fetch(gn, lv);
+          pan_fetch(gn);
+          nxt = src[(int64_t)gn * (2 * kTileS)];
+        }
+        const uint32_t word = (s4 < 2) ? cur.x : cur.y;
+        fp_u32x4 xh, xl;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t c = (word >> (16 * (s4 & 1) + 4 * q)) & 15u;
+          const uint32_t idx = (uint32_t)(((s4 * 4 + q) * 2) * 16) + 16u * (uint32_t)h + c;
+          xh[q] = th[idx];
+          xl[q] = tl[idx];
+        }
+        const fp_bf16x8 ah = __builtin_bit_cast(fp_bf16x8, xh), al = __builtin_bit_cast(fp_bf16x8, xl);
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+          const fp_bf16x8 bh = *(const fp_bf16x8 *)&sB[buf][0][32 * ct + r][16 * s4 + 8 * h];
+          const fp_bf16x8 bl = *(const fp_bf16x8 *)&sB[buf][1][32 * ct + r][16 * s4 + 8 * h];
+          acc[ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bl, acc[ct], 0, 0, 0);
+          acc[ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[ct], 0, 0, 0);
+          acc[ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[ct], 0, 0, 0);
+          acc[ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[ct], 0, 0, 0);
+        }
+      }
+      if (--left == 0) {
+        promote();
+        left = blkg;
+      }
+      tab_store(buf ^ 1, lv);
+      pan_store(buf ^ 1);
+      cur = nxt;
+      __syncthreads();
+    }
+    promote();
+    double *out = part + ((int64_t)s * npad + (int64_t)st * kTileS) * ldp;
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int row = 32 * w + 8 * (q >> 2) + 4 * h + (q & 3);
+        out[(int64_t)row * ldp + 64 * cg + 32 * ct + r] = dacc[ct][q];
+      }
+  }
 }
 
 // ---------------------------------------------------------------- Jacobi
@@ -696,10 +1004,77 @@ __global__ void k_scale_cols(double *__restrict__ V, int32_t n, int32_t r,
   V[idx] *= s[idx / n];
 }
 
+// X A in the split-BF16 form: A (npad x lda, rows = samples) split into th/tl
+// [cxa][npad], then the persistent product into Y (rows = SNPs, ld ldy)
+static int launch_xa_bf16(eig_ctx *ctx, const Plan &P, const uint32_t *lbf, uint16_t *th,
+                          uint16_t *tl, int32_t cxa, const double *A, int32_t lda, double *Y,
+                          int64_t ldy, int32_t C, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_fpca_split_t, dim3((unsigned)(P.npad / 32), (unsigned)(cxa / 32)),
+                     dim3(256), 0, st, A, (int64_t)P.npad, (int64_t)lda, C, (int64_t)P.npad, cxa,
+                     th, tl);
+  if (cxa == 32)
+    hipLaunchKernelGGL(k_fpca_xa_bf16<1>, dim3(grid), dim3(256), 0, st, P.sel, P.M, P.nst, lbf,
+                       (const uint16_t *)th, (const uint16_t *)tl, (int64_t)P.npad, Y, ldy, C, 4);
+  else
+    hipLaunchKernelGGL(k_fpca_xa_bf16<2>, dim3(grid), dim3(256), 0, st, P.sel, P.M, P.nst, lbf,
+                       (const uint16_t *)th, (const uint16_t *)tl, (int64_t)P.npad, Y, ldy, C, 4);
+  EIG_HIP(ctx, hipGetLastError());
+  return EIG_OK;
+}
+
+// scale * X^T B in the split-BF16 form: B (M x ldb, rows = SNPs) split into th/tl
+// [64 ncg][Mpad]; the SNP range is cut into S chunks (multiples of 64) chosen like the
+// XTX's split count; the partials are summed in split order into out (ld ldo)
+static int launch_xtb_bf16(eig_ctx *ctx, const Plan &P, const uint2 *selg, const uint32_t *lbf,
+                           uint16_t *th, uint16_t *tl, int64_t Mpad, const double *B, int64_t ldb,
+                           int32_t C, double *part, size_t part_doubles, double *out, int32_t ldo,
+                           double scale, int64_t slots, hipStream_t st) {
+  const int32_t ncg = (C + 63) / 64, cp = 64 * ncg;
+  hipLaunchKernelGGL(k_fpca_split_t, dim3((unsigned)(Mpad / 32), (unsigned)(cp / 32)), dim3(256),
+                     0, st, B, P.M, ldb, C, Mpad, cp, th, tl);
+  const int64_t G64 = Mpad / kGrp;
+  const int64_t smax = std::max<int64_t>(
+      1, std::min<int64_t>({64, G64, (int64_t)(part_doubles / ((size_t)P.npad * cp))}));
+  int64_t chunk = Mpad, nsplit = 1;
+  double best = -1.0;
+  for (int64_t sp = 1; sp <= smax; ++sp) {
+    int64_t c = (P.M + sp - 1) / sp;
+    c = ((c + kGrp - 1) / kGrp) * kGrp;
+    const int64_t ns = (P.M + c - 1) / c;
+    if (ns != sp) continue;
+    const double rounds = (double)((P.nst * ns * ncg + slots - 1) / slots);
+    const double cost = rounds * (double)c / (double)P.M + 0.001 * (double)ns;
+    if (best < 0.0 || cost < best - 1e-12) {
+      best = cost;
+      chunk = c;
+      nsplit = ns;
+    }
+  }
+  const int64_t items = (int64_t)P.nst * nsplit * ncg;
+  const unsigned grid = (unsigned)std::min<int64_t>(items, slots);
+  hipLaunchKernelGGL(k_fpca_xtb_bf16, dim3(grid), dim3(256), 0, st, selg, G64, P.M, P.nst, lbf,
+                     (const uint16_t *)th, (const uint16_t *)tl, Mpad, ncg, (int32_t)nsplit, chunk,
+                     8, part, cp);
+  const int64_t tot = (int64_t)P.npad * cp;
+  hipLaunchKernelGGL(k_fpca_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                     (const double *)part, (int32_t)nsplit, (int64_t)P.npad, cp, scale, out, ldo);
+  EIG_HIP(ctx, hipGetLastError());
+  return EIG_OK;
+}
+
 extern "C" int eig_fpca(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint64_t seed,
                         const double *omega, double *evals, double *evecs) {
+  return eig_fpca_form(ctx, K, L, I, seed, omega, EIG_FORM_FP64, evals, evecs);
+}
+
+extern "C" int eig_fpca_form(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint64_t seed,
+                             const double *omega, int32_t form, double *evals, double *evecs) {
   if (!ctx || K <= 0 || L <= K || I <= 0 || !evals)
     return eig_set_err(ctx, EIG_E_ARG, "eig_fpca: need 0 < K < L and I > 0");
+  if (form != EIG_FORM_FP64 && form != EIG_FORM_BF16X4)
+    return eig_set_err(ctx, EIG_E_ARG, "eig_fpca: form must be FP64 or BF16X4");
+  const bool bf = form == EIG_FORM_BF16X4;
+  if (bf && L > 64) return eig_set_err(ctx, EIG_E_ARG, "eig_fpca: the BF16X4 form needs L <= 64");
   const int32_t Ctot = (I + 1) * L;
   if (L > 16 * kMaxCT || Ctot > kJacMax)
     return eig_set_err(ctx, EIG_E_ARG, "eig_fpca: L <= %d and (I+1)*L <= %d", 16 * kMaxCT,
@@ -740,6 +1115,13 @@ extern "C" int eig_fpca(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint64_t 
   const size_t o_sm = add(sizeof(double) * (kJacMax + 64) + sizeof(int32_t) * (kJacMax + 1));
   const size_t o_cb = add(sizeof(double) * kJacMax * kJacMax);
   const size_t o_ev = add(sizeof(double) * (size_t)K * npad);
+  // split-BF16 form: levels, the X A panel (32 * ceil(L/32) columns) and the X^T B
+  // panel (64 * ceil(C/64) columns, C <= (I+1)L) as transposed bf16 hi / lo
+  const int64_t Mpad = ((M + kGrp - 1) / kGrp) * kGrp;
+  const int32_t cxa = 32 * ((L + 31) / 32), cxb = 64 * ((Ctot + 63) / 64);
+  const size_t o_lbf = bf ? add(sizeof(uint32_t) * 4 * M) : 0;
+  const size_t o_ta = bf ? add(sizeof(uint16_t) * 2 * (size_t)cxa * npad) : 0;
+  const size_t o_tb = bf ? add(sizeof(uint16_t) * 2 * (size_t)cxb * Mpad) : 0;
   void *old = ctx->scratch2.p;
   EIG_TRY(eig_ensure(ctx, ctx->scratch2, off));
   char *base = (char *)ctx->scratch2.p;
@@ -750,6 +1132,9 @@ extern "C" int eig_fpca(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint64_t 
   int32_t *d_sweeps = (int32_t *)(sm + kJacMax);
   double *Cb = (double *)(base + o_cb);
   const size_t part_doubles = (size_t)Smax * npad * 16 * kMaxCT;
+  uint32_t *lbf = bf ? (uint32_t *)(base + o_lbf) : nullptr;
+  uint16_t *tah = bf ? (uint16_t *)(base + o_ta) : nullptr, *tal = tah ? tah + (size_t)cxa * npad : nullptr;
+  uint16_t *tbh = bf ? (uint16_t *)(base + o_tb) : nullptr, *tbl = tbh ? tbh + (size_t)cxb * Mpad : nullptr;
 
   if (!F.blas) {
     FP_BLAS(ctx, rocblas_create_handle(&F.blas));
@@ -774,23 +1159,51 @@ extern "C" int eig_fpca(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint64_t 
   }
   EIG_HIP(ctx, hipGetLastError());
   Plan P{M, nst, npad, (const uint32_t *)ctx->sel.p, lf};
-  // the tuners write scratch outputs (U and Bm), never Q or the panels in use
-  EIG_TRY(tune_xa(ctx, F, P, CTL, A0, lda, U, ldq));
-  EIG_TRY(tune_xtb(ctx, F, P, CTL, U, ldq, part, Smax, Bm, lda));
+  int64_t bslots = 0;
+  int bxa_grid = 0;
+  if (bf) {
+    hipLaunchKernelGGL(k_fpca_levbf, dim3((unsigned)((4 * M + 255) / 256)), dim3(256), 0, st,
+                       (const double *)lf, M, lbf);
+    EIG_HIP(ctx, hipGetLastError());
+    EIG_TRY(eig_ensure_selg(ctx));
+    int occ = 0;
+    EIG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fpca_xtb_bf16, 256, 0));
+    bslots = (int64_t)std::max(occ, 1) * std::max(ctx->num_cu, 1);
+    occ = 0;
+    EIG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fpca_xa_bf16<2>, 256, 0));
+    bxa_grid = (int)std::min<int64_t>((M + 127) / 128, (int64_t)std::max(occ, 1) * ctx->num_cu);
+  } else {
+    // the tuners write scratch outputs (U and Bm), never Q or the panels in use
+    EIG_TRY(tune_xa(ctx, F, P, CTL, A0, lda, U, ldq));
+    EIG_TRY(tune_xtb(ctx, F, P, CTL, U, ldq, part, Smax, Bm, lda));
+  }
+  auto xa = [&](const double *A_, double *Y_) -> int {
+    if (bf)
+      return launch_xa_bf16(ctx, P, lbf, tah, tal, cxa, A_, lda, Y_, ldq, L, bxa_grid, st);
+    return launch_xa(ctx, P, CTL, F.xa[CTL].grid, A_, lda, Y_, ldq, L, st);
+  };
+  auto xtb = [&](const double *B_, double *out_) -> int {
+    if (bf)
+      return launch_xtb_bf16(ctx, P, (const uint2 *)ctx->selg.p, lbf, tbh, tbl, Mpad, B_, ldq, L,
+                             part, part_doubles, out_, lda, 1.0 / (double)ncols, bslots, st);
+    return launch_xtb(ctx, P, CTL, F.xtb[CTL].grid, F.xtb[CTL].S, B_, ldq, L, part, out_, lda,
+                      1.0 / (double)ncols, st);
+  };
   // ---- power iterations (captured once per shape and buffer set, then replayed)
   const void *key[4] = {ctx->sel.p, lf, Q, part};
-  const int64_t shape[5] = {M, nst, L, I, (int64_t)F.xa[CTL].grid * 1000 + F.xtb[CTL].S};
+  const int64_t shape[5] = {M, nst, L, I,
+                            bf ? -(int64_t)ctx->num_cu
+                               : (int64_t)F.xa[CTL].grid * 1000 + F.xtb[CTL].S};
   const bool same = F.gexec && std::equal(key, key + 4, F.gkey) &&
                     std::equal(shape, shape + 5, F.gshape) && old == ctx->scratch2.p;
   auto power = [&]() -> int {
     double *cur = A0, *nxt = A1;
     for (int32_t i = 0; i < I; ++i) {
-      EIG_TRY(launch_xa(ctx, P, CTL, F.xa[CTL].grid, cur, lda, Q + (size_t)i * L, ldq, L, st));
-      EIG_TRY(launch_xtb(ctx, P, CTL, F.xtb[CTL].grid, F.xtb[CTL].S, Q + (size_t)i * L, ldq, L,
-                         part, nxt, lda, 1.0 / (double)ncols, st));
+      EIG_TRY(xa(cur, Q + (size_t)i * L));
+      EIG_TRY(xtb(Q + (size_t)i * L, nxt));
       std::swap(cur, nxt);
     }
-    EIG_TRY(launch_xa(ctx, P, CTL, F.xa[CTL].grid, cur, lda, Q + (size_t)I * L, ldq, L, st));
+    EIG_TRY(xa(cur, Q + (size_t)I * L));
     return EIG_OK;
   };
   if (getenv("EIG_NO_GRAPH")) {
@@ -880,9 +1293,14 @@ extern "C" int eig_fpca(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint64_t 
   if (r < K) return eig_set_err(ctx, EIG_E_NUMERIC, "fpca: Krylov basis rank %d < K", r);
   // ---- B = X^T U  (npad x r, ld ldq), then the top-K left singular vectors of B
   const int CTr = (r + 15) / 16;
+  if (bf) {
+    EIG_TRY(launch_xtb_bf16(ctx, P, (const uint2 *)ctx->selg.p, lbf, tbh, tbl, Mpad, src, ldq, r,
+                            part, part_doubles, Bm, (int32_t)ldq, 1.0, bslots, st));
+  } else {
   EIG_TRY(tune_xtb(ctx, F, P, CTr, src, ldq, part, Smax, Bm, (int32_t)ldq));
   EIG_TRY(launch_xtb(ctx, P, CTr, F.xtb[CTr].grid, F.xtb[CTr].S, src, ldq, r, part, Bm,
                      (int32_t)ldq, 1.0, st));
+  }
   EIG_TRY(gemm_nt_long(ctx, F.blas, r, r, npad, Bm, ldq, Bm, ldq, G, r, part, part_doubles, st));
   // top-K eigenpairs of B^T B (order r up to 256) by the FP64 eigensolver: all
   // eigenvalues, K eigenvectors into V (column-major r x K)

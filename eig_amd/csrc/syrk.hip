@@ -144,7 +144,6 @@ __global__ __launch_bounds__(256, 2) void k_syrk_f64(const uint32_t *__restrict_
 // hold the code of SNP 64 g + 16 s + 8 h + j (s = 16-SNP stage 0..3, h = half of
 // the stage 0..1, j = 0..7); SNPs past M carry code 3.  A lane of the SYRK that
 // feeds MFMA k-slots 8h..8h+7 of a sample reads one 8-byte word pair per group.
-constexpr int kGrp = 64;
 
 __global__ __launch_bounds__(256) void k_regroup(const uint32_t *__restrict__ sel, int64_t M,
                                                  int64_t G64, int64_t ngroups,
@@ -412,6 +411,22 @@ __global__ __launch_bounds__(256) void k_unpack_scale(const double *__restrict__
   }
 }
 
+// the per-64-SNP code layout of the current selection (built once per selection)
+int eig_ensure_selg(eig_ctx *ctx) {
+  if (ctx->have_selg) return EIG_OK;
+  const int64_t M = ctx->nsnp;
+  const int64_t G64 = (M + kGrp - 1) / kGrp;
+  const int64_t ngroups = (int64_t)ctx->nst * G64;
+  EIG_TRY(eig_ensure(ctx, ctx->selg, (size_t)ngroups * kTileS * 16));
+  eig_time_begin(ctx);
+  hipLaunchKernelGGL(k_regroup, dim3((unsigned)((ngroups + 1) / 2)), dim3(256), 0, ctx->stream,
+                     (const uint32_t *)ctx->sel.p, M, G64, ngroups, (uint4 *)ctx->selg.p);
+  EIG_HIP(ctx, hipGetLastError());
+  eig_time_end(ctx, "regroup");
+  ctx->have_selg = true;
+  return EIG_OK;
+}
+
 int eig_run_syrk(eig_ctx *ctx, int form, int bf16_block, double *trace_scale) {
   const int64_t M = ctx->nsnp;
   const int32_t T = ctx->nst;
@@ -465,16 +480,7 @@ int eig_run_syrk(eig_ctx *ctx, int form, int bf16_block, double *trace_scale) {
   double *d_tr = (double *)ctx->part.p + tile * P * nsplit;
   unsigned grid = (unsigned)(P * nsplit);
   const int64_t G64 = (M + kGrp - 1) / kGrp;
-  if (split && !ctx->have_selg) {
-    const int64_t ngroups = (int64_t)T * G64;
-    EIG_TRY(eig_ensure(ctx, ctx->selg, (size_t)ngroups * kTileS * 16));
-    eig_time_begin(ctx);
-    hipLaunchKernelGGL(k_regroup, dim3((unsigned)((ngroups + 1) / 2)), dim3(256), 0, ctx->stream,
-                       (const uint32_t *)ctx->sel.p, M, G64, ngroups, (uint4 *)ctx->selg.p);
-    EIG_HIP(ctx, hipGetLastError());
-    eig_time_end(ctx, "regroup");
-    ctx->have_selg = true;
-  }
+  if (split) EIG_TRY(eig_ensure_selg(ctx));
   eig_time_begin(ctx);
   if (form == EIG_FORM_FP64) {
     hipLaunchKernelGGL(k_syrk_f64, dim3(grid), dim3(256), 0, ctx->stream,

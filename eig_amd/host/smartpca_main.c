@@ -391,7 +391,9 @@ int main(int argc, char **argv) {
     CHK(eig_snpstats(ctx, live, S.altnormstyle, S.fancynorm, S.minallelecnt, S.maxmissing, NULL,
                      NULL, NULL, NULL, NULL, NULL));
     double *fev = malloc(sizeof(double) * K), *fvec = malloc(sizeof(double) * K * (size_t)nrows);
-    CHK(eig_fpca(ctx, K, L, I, (uint64_t)S.seed, NULL, fev, fvec));
+    /* xtxform: bf16x4 selects the split-BF16 panel products (fp64 otherwise) */
+    CHK(eig_fpca_form(ctx, K, L, I, (uint64_t)S.seed, NULL,
+                      form == EIG_FORM_BF16X4 ? EIG_FORM_BF16X4 : EIG_FORM_FP64, fev, fvec));
     if (S.verbose) printf("fastmode: %.3f s on the device\n", now_s() - t0);
     /* printevecs (smartpca.c:3811-3875): header, then rows only when every
        individual is an axis sample */

@@ -144,6 +144,14 @@ int eig_lsqproject(eig_ctx *ctx, const int32_t *use, double *acoeffs, double *bc
 int eig_fpca(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint64_t seed, const double *omega,
              double *evals, double *evecs);
 
+/* eig_fpca with the panel-product form chosen: EIG_FORM_FP64 (as eig_fpca) or
+ * EIG_FORM_BF16X4 -- genotype levels and the dense panels split into bf16 hi + lo,
+ * four MFMA products in FP32 within 512-sample / 512-SNP blocks, FP64 across blocks
+ * (L <= 64).  Same kjg_fpca call (ksrc/kjg_fpca.c:24-101); only the precision of the
+ * products X A and X^T B (kjg_fpca.c:104-178) differs. */
+int eig_fpca_form(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint64_t seed,
+                  const double *omega, int32_t form, double *evals, double *evecs);
+
 /* ---- lsqproject serving ------------------------------------------------------
  * The reference projects samples once per run, inside lsqproj (smartpca.c:
  * 4606-4757); the server keeps that step's inputs (loadings, fxscal, fancy, xmean,

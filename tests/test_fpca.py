@@ -96,6 +96,47 @@ def test_fpca_builtin_generator_reaches_exact_eigenpairs(eng, N, M, K, L, I):
     assert np.array_equal(ev, ev2) and np.array_equal(vec, vec2)
 
 
+@pytest.mark.parametrize("N,M,K,L,I", [(300, 20000, 10, 20, 10), (200, 4000, 3, 8, 5)])
+def test_fpca_split_bf16_panels_match_fp64(eng, N, M, K, L, I):
+    """The split-BF16 panel products (4 products, FP32 within 512-sample/SNP blocks)
+    against the FP64 form with the same start panel.  Bars: eigenvalues 1e-6
+    relative, eigenvectors 1e-5."""
+    import eig_amd
+    g = _panel(N + 1, M, 12, N // 12 + 1)[:, :N]
+    rows = O.pack_genotypes(g)
+    xindex = np.arange(N, dtype=np.int32)
+    _say(f"fpca bf16 vs fp64 N={N} M={M} K={K} L={L} I={I}")
+    eng.load_packed(rows, N)
+    eng.select(xindex)
+    eng.snpstats()
+    ev64, v64 = eng.fpca(K, L, I, seed=99, form=eig_amd.FORM_FP64)
+    ev16, v16 = eng.fpca(K, L, I, seed=99, form=eig_amd.FORM_BF16X4)
+    assert np.allclose(ev16, ev64, rtol=1e-6), (ev16, ev64)
+    assert np.max(np.abs(v16 - v64)) < 1e-5
+    ev16b, v16b = eng.fpca(K, L, I, seed=99, form=eig_amd.FORM_BF16X4)
+    assert np.array_equal(ev16, ev16b) and np.array_equal(v16, v16b)
+
+
+def test_fastmode_selection_above_the_grm_limit(eng):
+    """fastmode takes more samples than the N x N path allows (the reference checks
+    nrows < 46341 only outside fastmode, smartpca.c:955-988)."""
+    import eig_amd
+    N, M = 46400, 256
+    rng = np.random.default_rng(5)
+    p = rng.uniform(0.1, 0.5, size=M)
+    g = rng.binomial(2, p[:, None], size=(M, N)).astype(np.int32)
+    rows = O.pack_genotypes(g)
+    _say(f"fastmode above the GRM limit N={N} M={M}")
+    eng.load_packed(rows, N)
+    eng.select(np.arange(N, dtype=np.int32))
+    eng.snpstats()
+    with pytest.raises(eig_amd.EigError, match="GRM"):
+        eng.xtx(0, 0, want_matrix=False)
+    ev, vec = eng.fpca(2, 4, 2, seed=3)
+    assert np.all(np.isfinite(ev)) and ev[0] > 0 and vec.shape == (2, N)
+    assert np.allclose(np.sum(vec * vec, axis=1), 1.0, atol=1e-12)
+
+
 def test_fpca_rejects_bad_shapes(eng):
     import eig_amd
     g = _panel(1, 500, 2, 20)
