@@ -164,6 +164,44 @@ def serve_leg(eng, P, vec, seconds, rate=1000.0, pool_n=4096):
                 chunk=info0["chunk"], grid=info0["grid"])
 
 
+def fastmode_leg(args):
+    """fastmode at the 50-population panel's shape (N=100,000 x M=500,000, 1% missing;
+    --scale / --nsnp shrink it): one warm-up and `steps` timed calls per form."""
+    import eig_amd
+    from eig_amd import synth
+    P = synth.config_panel(4, nsnp=None if args.nsnp == 600000 else args.nsnp, scale=args.scale)
+    N, M = P.nind, P.nsnp
+    log(f"fastmode: generating N={N} M={M} in HBM")
+    eng = eig_amd.Engine(0)
+    P.to_device(eng)
+    eng.select(np.arange(N, dtype=np.int32))
+    eng.snpstats()
+    out = {"metric": "fastmode s per call (K=10, L=20, I=10)", "unit": "s", "n_gpus": 1,
+           "config": {"workload": "fastmode randomized PCA, 50 Balding-Nichols pops, 1% missing",
+                      "N": N, "M": M, "K": 10, "L": 20, "I": 10}, "forms": {}}
+    ref = None
+    for form, name in ((eig_amd.FORM_FP64, "fp64"), (eig_amd.FORM_BF16X4, "bf16x4")):
+        for _ in range(args.warmup):
+            eng.fpca(10, 20, 10, seed=1234, form=form)
+        times = []
+        for s in range(args.steps):
+            t0 = time.perf_counter()
+            ev, vec = eng.fpca(10, 20, 10, seed=1234, form=form)
+            times.append(time.perf_counter() - t0)
+            log(f"fastmode {name} call {s}: {times[-1]:.3f} s (power {eng.last_ms('fpca_power'):.1f} ms, "
+                f"finish {eng.last_ms('fpca_finish'):.1f} ms)")
+        rec = {"s": round(float(np.mean(times)), 4), "power_ms": eng.last_ms("fpca_power"),
+               "finish_ms": eng.last_ms("fpca_finish"), "evals": [round(float(x), 6) for x in ev]}
+        if ref is None:
+            ref = (ev, vec)
+        else:
+            rec["eval_relerr_vs_fp64"] = float(np.max(np.abs(ev - ref[0]) / np.abs(ref[0])))
+            rec["evec_maxabs_vs_fp64"] = float(np.max(np.abs(vec - ref[1])))
+        out["forms"][name] = rec
+    eng.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,6 +215,9 @@ def main():
     ap.add_argument("--forms", default="bf16x4,bf16x3,fp64",
                     help="forms to run; anything but all three prints stage times only "
                          "(for profiler runs)")
+    ap.add_argument("--fastmode", action="store_true",
+                    help="instead of the XTX pass, time fastmode (K=10, L=20, I=10) on the "
+                         "100,000 x 500,000 panel in both panel-product forms")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="first copy this many MiB device-to-device (the known-size copy "
                          "that scales the HBM byte counters of a --pmc run)")
@@ -187,6 +228,9 @@ def main():
         sys.exit(2)
     import eig_amd
     from eig_amd import synth
+    if args.fastmode:
+        fastmode_leg(args)
+        return
     nvec = 10
     P = synth.config_panel(2, nsnp=args.nsnp, scale=args.scale)
     N, M = P.nind, P.nsnp
