@@ -3,18 +3,21 @@
 //  1. Householder tridiagonalisation, blocked by panels of kNB columns (the
 //     dsytrd/dlatrd scheme).  The working matrix is row-major and symmetric with
 //     a padded leading dimension, so "column c below the diagonal" is row c right
-//     of the diagonal and every access is a contiguous 16-byte pair.  Two kernels
-//     run per column c (panel position j):
-//       k_symv : the reflector scalars of column c (dlarfg), then y = A_trail v
-//                with v rebuilt on the fly from the unscaled column x; per-block
-//                partial y.v;
-//       k_wcol : w = tau (y - V q - W p) + alpha v, where alpha = -tau/2 w.v is
-//                known up front from w.v = tau (y.v - 2 p.q); in the same pass the
-//                next row x = A[c+1] - (V W^T + W V^T)[c+1] and per-block partials
-//                of its norm and of the next column's panel dots V^T x, W^T x.
-//     No block waits for another: each kernel reduces the previous kernel's
-//     per-block partials in block order (same bits in every block, independent of
-//     scheduling), so the sequence needs no atomics or fences.  After each panel
+//     of the diagonal and every access is a contiguous 16-byte pair.  Each column c
+//     (panel position j) runs in two parts:
+//       symv : the reflector scalars of column c (dlarfg), then y = A_trail v
+//              with v rebuilt on the fly from the unscaled column x; per-block
+//              partial y.v;
+//       wcol : w = tau (y - V q - W p) + alpha v, where alpha = -tau/2 w.v is
+//              known up front from w.v = tau (y.v - 2 p.q); in the same pass the
+//              next row x = A[c+1] - (V W^T + W V^T)[c+1] and per-block partials
+//              of its norm and of the next column's panel dots V^T x, W^T x.
+//     Trailing orders of 2,000 and more run the parts as two kernels (k_symv,
+//     k_wcol); shorter ones run them in one launch, k_column, whose w-column
+//     blocks load what they can up front and wait on a ticket counter for the
+//     symv blocks.
+//     Cross-block sums are per-block partials reduced in a fixed order (same
+//     bits in every block, independent of scheduling).  After each panel
 //     k_syr2k applies A -= V W^T + W V^T to the trailing matrix.  The reflector of
 //     row c is left in A[c][c+1:], with tau[c].  The launch sequence for one n is
 //     captured once into a hipGraph and replayed.
@@ -50,7 +53,8 @@ struct EigWs {
   double *sc;
   double *part1;          // per-block partial dots of the back-transform
   double *pw;             // [2][maxw][kPW] column hand-off partials (by column parity)
-  double *ps;             // [maxblk] per-block y.v of k_symv
+  double *ps;             // [maxblk] per-block y.v of the symv part
+  int32_t *sync;          // [0] hand-off ticket counter, [1] hand-off timeout flag
   int32_t maxw;
   double *eval;           // ascending
   double *X;              // nvec x n eigenvectors
@@ -68,22 +72,35 @@ __device__ inline double block_sum(double v, double *red) {
   return (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-// ---- hand-off between the column kernels -----------------------------------
-// No kernel of the column sequence waits for another block: every cross-block sum
-// is written as per-block partials and reduced, in block order, by the blocks of
-// the next kernel (the kernel boundary makes the partials visible).
-//   k_wcol(c-1) -> pw[c & 1][b][kPW]: sum x^2 over rows >= c+2, x_{c+1} (alpha,
-//                  from block 0 only), and for q below column c's panel position
-//                  sum_{r >= c+2} V_q[r] x_r and sum W_q[r] x_r;
-//   k_symv(c)   -> ps[b]: sum over its rows of y_r v_r.
+// ---- partials handed from part to part -------------------------------------
+// Every cross-block sum is written as per-block partials and reduced, in block
+// order, by the blocks of the next part:
+//   wcol(c-1) -> pw[c & 1][b][kPW]: sum x^2 over rows >= c+2, x_{c+1} (alpha,
+//                from block 0 only), and for q below column c's panel position
+//                sum_{r >= c+2} V_q[r] x_r and sum W_q[r] x_r;
+//   symv(c)   -> ps[b]: sum over its rows of y_r v_r.
 constexpr int kPW = 2 + 2 * kNB;
 
 // sum_{b = start, start+step, ... < end} p[b * stride], in that order, with the
-// loads issued eight at a time so that their latencies overlap
-__device__ inline double sum_strided(const double *__restrict__ p, int64_t stride, int32_t start,
-                                     int32_t step, int32_t end) {
+// loads issued eight at a time so that their latencies overlap.  The first eight
+// (load8) are straight-line and unconditional (indices past the end read entry
+// end-1 and are discarded), so a caller can issue them together with its other
+// loads and add them up later (sum_rest).  Needs start >= 0 and end >= 1.
+__device__ inline void load8(const double *__restrict__ p, int64_t stride, int32_t start,
+                             int32_t step, int32_t end, double (&v)[8]) {
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int32_t bb = start + u * step;
+    const double x = p[(int64_t)min(bb, end - 1) * stride];
+    v[u] = bb < end ? x : 0.0;
+  }
+}
+__device__ inline double sum_rest(const double *__restrict__ p, int64_t stride, int32_t start,
+                                  int32_t step, int32_t end, const double (&v0)[8]) {
   double s = 0.0;
-  for (int32_t b = start; b < end; b += 8 * step) {
+#pragma unroll
+  for (int u = 0; u < 8; ++u) s += v0[u];
+  for (int32_t b = start + 8 * step; b < end; b += 8 * step) {
     double v[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
@@ -95,19 +112,24 @@ __device__ inline double sum_strided(const double *__restrict__ p, int64_t strid
   }
   return s;
 }
+__device__ inline double sum_strided(const double *__restrict__ p, int64_t stride, int32_t start,
+                                     int32_t step, int32_t end) {
+  if (end <= 0) return 0.0;
+  double v[8];
+  load8(p, stride, start, step, end, v);
+  return sum_rest(p, stride, start, step, end, v);
+}
 
 __device__ inline int32_t wcol_blocks(int32_t n, int32_t c) {  // producer blocks of column c
   return c == 0 ? 1 : (n - c + 255) / 256;
 }
 
 // dlarfg scalars of column c from the producers' partials, reduced by one wave in
-// a fixed order (the same bits in every block that calls it).
-__device__ inline void col_scalars(const EigWs &ws, int32_t n, int32_t c, double &tau,
-                                   double &scale, double &beta) {
-  const double *pw = ws.pw + (int64_t)(c & 1) * ws.maxw * kPW;
-  const int32_t nw = wcol_blocks(n, c);
-  const int lane = threadIdx.x & 63;
-  double s0 = sum_strided(pw, kPW, lane, 64, nw), s1 = sum_strided(pw + 1, kPW, lane, 64, nw);
+// a fixed order (the same bits in every wave that calls it).  col_scalars_sums
+// takes each lane's strided sums of the partials (sum over b = lane, lane+64, ...
+// of pw[b][0] and pw[b][1]).
+__device__ inline void col_scalars_sums(double s0, double s1, double &tau, double &scale,
+                                        double &beta) {
   for (int o = 32; o > 0; o >>= 1) {
     s0 += __shfl_down(s0, o, 64);
     s1 += __shfl_down(s1, o, 64);
@@ -124,6 +146,14 @@ __device__ inline void col_scalars(const EigWs &ws, int32_t n, int32_t c, double
     scale = 1.0 / (alpha - beta);
   }
 }
+__device__ inline void col_scalars(const EigWs &ws, int32_t n, int32_t c, double &tau,
+                                   double &scale, double &beta) {
+  const double *pw = ws.pw + (int64_t)(c & 1) * ws.maxw * kPW;
+  const int32_t nw = wcol_blocks(n, c);
+  const int lane = threadIdx.x & 63;
+  col_scalars_sums(sum_strided(pw, kPW, lane, 64, nw), sum_strided(pw + 1, kPW, lane, 64, nw),
+                   tau, scale, beta);
+}
 
 // column 0: x = row 0, d[0], and the partials of column 0 (one block)
 __global__ __launch_bounds__(256) void k_col0(EigWs ws, int32_t n) {
@@ -139,18 +169,27 @@ __global__ __launch_bounds__(256) void k_col0(EigWs ws, int32_t n) {
     ws.d[0] = ws.a[0];
     ws.pw[0] = s;
     ws.pw[1] = n > 1 ? ws.a[1] : 0.0;
+    ws.sync[0] = 0;
+    ws.sync[1] = 0;
   }
 }
 
-// ---------------------------------------------------------------- k_symv
+// a store that is coherent device-wide once the wave's vmcnt has drained (it
+// writes through the XCD's L2), for the values the wcol part of the same launch
+// reads (y and the y.v partials)
+__device__ inline void store_dev(double *p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------- symv part
 constexpr int kSymvRows = 8;  // rows per block
-__global__ __launch_bounds__(256) void k_symv(EigWs ws, int32_t n, int32_t c, int32_t j) {
+__device__ inline void symv_part(const EigWs &ws, int32_t n, int32_t c, int32_t j, int32_t blk) {
   __shared__ double red[4][kSymvRows];
   __shared__ double yv[kSymvRows];
   __shared__ double sscale;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int32_t lda = ws.lda;
-  const int32_t r0 = c + 1 + blockIdx.x * kSymvRows;
+  const int32_t r0 = c + 1 + blk * kSymvRows;
   const int32_t a0 = (c + 1) & ~1;
   const int32_t npair = (((n + 1) & ~1) - a0) >> 1;
   const double2 *xp = (const double2 *)(ws.xtmp + a0);
@@ -181,7 +220,7 @@ __global__ __launch_bounds__(256) void k_symv(EigWs ws, int32_t n, int32_t c, in
     col_scalars(ws, n, c, tau, scale, beta);
     if (lane == 0) {
       sscale = scale;
-      if (blockIdx.x == 0) {
+      if (blk == 0) {
         ws.e[c] = beta;
         ws.e2[c] = beta * beta;
         ws.tau[c] = tau;
@@ -230,11 +269,21 @@ __global__ __launch_bounds__(256) void k_symv(EigWs ws, int32_t n, int32_t c, in
 #pragma unroll
     for (int rr = 0; rr < kSymvRows; ++rr) acc[rr] += a0v[rr].x * v0 + a0v[rr].y * v1;
   }
+  // butterfly transpose-reduce of the 8 row sums: after the exchanges at offsets 4,
+  // 2, 1 lane l holds row (l & 7) summed over its group of 8 lanes; three more
+  // exchanges sum the groups
+  static_assert(kSymvRows == 8, "the reduction below is written for 8 rows");
 #pragma unroll
-  for (int rr = 0; rr < kSymvRows; ++rr) {
-    double a = acc[rr];
-    for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
-    if (lane == 0) red[wv][rr] = a;
+  for (int o = 4; o > 0; o >>= 1) {
+    const bool b = (lane & o) != 0;
+#pragma unroll
+    for (int k = 0; k < o; ++k) acc[k] = (b ? acc[k + o] : acc[k]) + __shfl_xor(b ? acc[k] : acc[k + o], o, 64);
+  }
+  {
+    double a = acc[0];
+#pragma unroll
+    for (int o = 8; o < 64; o <<= 1) a += __shfl_xor(a, o, 64);
+    if (lane < kSymvRows) red[wv][lane] = a;
   }
   __syncthreads();
   // per row: y, v into V[j] and A[c], and the row's y.v
@@ -246,7 +295,7 @@ __global__ __launch_bounds__(256) void k_symv(EigWs ws, int32_t n, int32_t c, in
       const double vr = (r == c + 1) ? 1.0 : ws.xtmp[r] * scale;
       const double yr = (red[0][rr] + red[1][rr]) + (red[2][rr] + red[3][rr]);
       val = yr * vr;
-      ws.y[r] = yr;
+      store_dev(ws.y + r, yr);
       ws.V[(int64_t)r * kNB + j] = vr;
       ws.a[(int64_t)c * lda + r] = vr;
     }
@@ -257,20 +306,21 @@ __global__ __launch_bounds__(256) void k_symv(EigWs ws, int32_t n, int32_t c, in
     double sum = 0.0;
 #pragma unroll
     for (int rr = 0; rr < kSymvRows; ++rr) sum += yv[rr];
-    ws.ps[blockIdx.x] = sum;
+    store_dev(ws.ps + blk, sum);
   }
 }
 
-// ---------------------------------------------------------------- k_wcol
+// ---------------------------------------------------------------- wcol part
 // Panel vectors are stored row-major, V[r * kNB + q], so a thread loads its row's
 // coefficients in one batch; entries of columns not yet in this panel are stale
 // but always meet a zero coefficient.
-__global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, int32_t j) {
+template <class Wait>
+__device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t j, int32_t blk,
+                                 Wait &&wait) {
   __shared__ double red[4][4];
   __shared__ double pqr[4][2 * kNB];  // raw panel dots per thread group
   __shared__ double pq[2 * kNB];      // p_q = V_q . v, q_q = W_q . v (zero for q >= j)
   __shared__ double cc[2 * kNB];      // V_q[c+1], W_q[c+1] (zero for q >= j)
-  __shared__ double sh[2];            // tau, scale
   __shared__ double tb[256][17];      // transpose buffer of the block reduction
   __shared__ double tb2[16][16];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -279,12 +329,19 @@ __global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, in
   const double *pwc = ws.pw + (int64_t)(c & 1) * ws.maxw * kPW;
   const int32_t nw = wcol_blocks(n, c);
   const int32_t gsym = (n - c - 1 + kSymvRows - 1) / kSymvRows;
-  // ---- every load that depends on nothing computed here is issued first: this
-  // thread's row (panel coefficients, y, the trailing row c+1), row c+1's panel
-  // coefficients and y, and the three reductions' partials
-  const int32_t r = c1 + blockIdx.x * 256 + threadIdx.x;
+  // ---- loads of data from earlier launches go out before the hand-off wait: this
+  // thread's row of the panel vectors (column j, written by this launch's symv part,
+  // is rebuilt from x below) and of the trailing row c+1, its x, row c+1's panel
+  // coefficients, and the panel-dot and norm partials
+  const int32_t r = c1 + blk * 256 + threadIdx.x;
   const bool inr = r < n;
   const int32_t rs = inr ? r : c1;  // rows past n read row c+1 (in bounds, discarded)
+  // first rounds of the three partial sums: panel dots (component lane over blocks
+  // wv, wv+4, ...), and the norm and alpha of column c (blocks lane, lane+64, ...)
+  double lp[8], ln0[8], ln1[8];
+  load8(pwc + 2 + lane, kPW, wv, 4, nw, lp);
+  load8(pwc, kPW, lane, 64, nw, ln0);
+  load8(pwc + 1, kPW, lane, 64, nw, ln1);
   double vrow[kNB], wrow[kNB];
   {
     const double2 *vp = (const double2 *)(ws.V + (int64_t)rs * kNB);
@@ -298,34 +355,36 @@ __global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, in
       wrow[2 * h + 1] = b2.y;
     }
   }
-  double yy = ws.y[rs];
   double base = ws.a[(int64_t)c1 * lda + rs];
-  const double yc1 = ws.y[c1];
+  const double xr = ws.xtmp[rs];
   double atc = 0.0;
   if (threadIdx.x < 2 * kNB) {
     const int q = threadIdx.x & (kNB - 1);
     atc = (threadIdx.x < kNB) ? ws.V[(int64_t)c1 * kNB + q] : ws.W[(int64_t)c1 * kNB + q];
   }
-  double yv = sum_strided(ws.ps, 1, threadIdx.x, 256, gsym);
-  double praw = 0.0;
+  double praw;
   {
     const int comp = lane;  // 0..31 V dots, 32..63 W dots
     const int q = comp & (kNB - 1);
-    if (q < j) praw = sum_strided(pwc + 2 + comp, kPW, wv, 4, nw);
+    praw = sum_rest(pwc + 2 + comp, kPW, wv, 4, nw, lp);
+    praw = q < j ? praw : 0.0;
   }
-  if (wv == 0) {
-    double tau, scale, beta;
-    col_scalars(ws, n, c, tau, scale, beta);
-    if (lane == 0) {
-      sh[0] = tau;
-      sh[1] = scale;
-    }
+  // every wave forms the scalars itself (same bits as the symv part's)
+  double tau, scale;
+  {
+    double beta;
+    col_scalars_sums(sum_rest(pwc, kPW, lane, 64, nw, ln0), sum_rest(pwc + 1, kPW, lane, 64, nw, ln1),
+                     tau, scale, beta);
   }
+  if (!wait()) return;
+  // ---- the symv part's outputs: y and the per-block partials of y.v
+  double yy = ws.y[rs];
+  const double yc1 = ws.y[c1];
+  double yv = sum_strided(ws.ps, 1, threadIdx.x, 256, gsym);
   for (int o = 32; o > 0; o >>= 1) yv += __shfl_down(yv, o, 64);
   if (lane == 0) red[wv][0] = yv;
   pqr[wv][lane] = praw;
   __syncthreads();
-  const double tau = sh[0], scale = sh[1];
   const double ydv = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
   if (threadIdx.x < 2 * kNB) {
     const int comp = threadIdx.x, q = comp & (kNB - 1);
@@ -350,10 +409,11 @@ __global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, in
     yy -= vrow[q] * pq[kNB + q] + wrow[q] * pq[q];
     base -= cc[q] * wrow[q] + cc[kNB + q] * vrow[q];
   }
-  // vrow[j]: this column's v; the thread's W[r][j] joins its registers too
-  double vr = 0.0;
+  // this column's v_r, as the symv part formed it, joins vrow at j; the thread's
+  // W[r][j] joins wrow below
+  const double vr = (r == c1) ? 1.0 : xr * scale;
 #pragma unroll
-  for (int q = 0; q < kNB; ++q) vr = (q == j) ? vrow[q] : vr;
+  for (int q = 0; q < kNB; ++q) vrow[q] = (q == j) ? vr : vrow[q];
   const double w = tau * yy;
   const double wj = w + alpha * vr;
   // row c+1 minus panel column j:  V[j][c1] (=1) W[j][r] + W[j][c1] V[j][r]
@@ -371,7 +431,7 @@ __global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, in
   // panel, V_q . x and W_q . x for q <= j
   const int jn = ((c1 % kNB) != 0 && c1 + 1 < n) ? j + 1 : 0;
   const double xl = live ? x : 0.0;
-  double *pwn = ws.pw + (int64_t)(c1 & 1) * ws.maxw * kPW + (int64_t)blockIdx.x * kPW;
+  double *pwn = ws.pw + (int64_t)(c1 & 1) * ws.maxw * kPW + (int64_t)blk * kPW;
   {
     double v0 = xl * xl, v1 = (inr && r == c1 + 1) ? x : 0.0;
     for (int o = 32; o > 0; o >>= 1) {
@@ -416,6 +476,70 @@ __global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, in
     pwn[0] = (red[0][1] + red[1][1]) + (red[2][1] + red[3][1]);
     pwn[1] = (red[0][2] + red[1][2]) + (red[2][2] + red[3][2]);
   }
+}
+
+// The two parts as kernels of their own, for long columns: there the symv part
+// streams a large trailing matrix and a launch boundary costs less than holding
+// w-column blocks and draining device-coherent stores inside one launch.
+__global__ __launch_bounds__(256) void k_symv(EigWs ws, int32_t n, int32_t c, int32_t j) {
+  symv_part(ws, n, c, j, (int32_t)blockIdx.x);
+}
+__global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, int32_t j) {
+  wcol_part(ws, n, c, j, (int32_t)blockIdx.x, []() { return true; });
+}
+
+// ---------------------------------------------------------------- k_column
+// One launch per column with gw + gsym blocks (gw = ceil((n-c-1)/256) w-column
+// chunks, gsym = ceil((n-c-1)/8) symv row groups).  Blocks gw .. gw+gsym-1 run the
+// symv part for their 8 rows and then add one to a ticket counter once their y and
+// ps stores (device-coherent stores) have completed.  Blocks 0 .. gw-1 run the
+// w-column part for one 256-row chunk each: they issue their loads of data from
+// earlier launches at once, then wait until all gsym tickets are in, then read y
+// and the y.v partials.  The waiters never hold up a symv block (there are at most
+// 182 of them, far fewer than the blocks the device holds at once), so the launch
+// cannot deadlock; the wait is bounded anyway, and a timeout sets sync[1] for the
+// host to report.  The waiters first sleep for quiet_ticks of the device's wall
+// clock (most of the symv part's expected duration), then poll.  The counter is cleared by k_col0 and grows monotonically over
+// the columns: column c's tickets are base .. base + gsym - 1.
+constexpr int64_t kHandoffSpin = 1 << 22;
+__global__ __launch_bounds__(256) void k_column(EigWs ws, int32_t n, int32_t c, int32_t j,
+                                                int32_t base, int64_t quiet_ticks) {
+  __shared__ int32_t ok_sh;
+  const int32_t gw = (n - c - 1 + 255) / 256;
+  const int32_t gsym = (int32_t)gridDim.x - gw;
+  if ((int32_t)blockIdx.x >= gw) {
+    symv_part(ws, n, c, j, (int32_t)blockIdx.x - gw);
+    // each wave drains its own device-coherent stores, then one thread adds the ticket
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (threadIdx.x == 0)
+      __hip_atomic_fetch_add(ws.sync, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  const uint64_t t_start = wall_clock64();
+  auto wait = [&]() -> bool {
+    if (threadIdx.x == 0) {
+      // no polling for most of the symv part's expected time: polls are device-
+      // coherent loads and slow down the symv blocks' stream of the matrix
+      while ((int64_t)(wall_clock64() - t_start) < quiet_ticks) __builtin_amdgcn_s_sleep(32);
+      int32_t ok = 0;
+      for (int64_t s = 0; s < kHandoffSpin; ++s) {
+        if (__hip_atomic_load(ws.sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= base + gsym) {
+          ok = 1;
+          break;
+        }
+        if (__hip_atomic_load(ws.sync + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+        __builtin_amdgcn_s_sleep(1);
+      }
+      if (!ok) __hip_atomic_store(ws.sync + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      ok_sh = ok;
+    }
+    __syncthreads();
+    if (!ok_sh) return false;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return true;
+  };
+  wcol_part(ws, n, c, j, (int32_t)blockIdx.x, wait);
 }
 
 // A[r][s] -= sum_q V[r][q] W[s][q] + W[r][q] V[s][q] for r, s >= k0: one FP64 MFMA
@@ -1006,15 +1130,35 @@ __global__ __launch_bounds__(256) void k_finish_vec(EigWs ws, int32_t n) {
 }
 
 // ---------------------------------------------------------------- driver
-static void launch_tridiag(EigWs ws, int32_t n, hipStream_t st) {
+// wall-clock ticks the w-column blocks of k_column sleep before polling: 80% of the
+// symv part's streaming time at 6.5 TB/s (its measured rate on large trailing
+// matrices), nothing below 2 us
+static int64_t quiet_ticks(int32_t L, int32_t clock_khz) {
+  const double us = 0.8 * (double)L * (double)L * 8.0 / 6.5e12 * 1e6;
+  return us < 2.0 ? 0 : (int64_t)(us * clock_khz / 1000.0);
+}
+
+// Trailing order below which k_column runs (one launch per column); longer columns
+// run k_symv + k_wcol.  Measured on MI355X: merging at every length costs ~10 us per
+// column on 5,000-long columns (the w-column blocks held for the whole stream and the
+// store drain) and saves ~3 us per column on short ones; 2,000 was the best cut for
+// n = 3,000 and 5,000 (1,000 to 4,000 tried).
+constexpr int32_t kMergeBelow = 2000;
+static void launch_tridiag(EigWs ws, int32_t n, int32_t clock_khz, hipStream_t st) {
   hipLaunchKernelGGL(k_col0, dim3(1), dim3(256), 0, st, ws, n);
+  int32_t base = 0;  // hand-off tickets taken by the earlier columns
   for (int32_t c = 0; c + 1 < n; ++c) {
     const int32_t j = c % kNB;
     const int32_t L = n - c - 1;
-    unsigned gsym = (unsigned)((L + kSymvRows - 1) / kSymvRows);
-    hipLaunchKernelGGL(k_symv, dim3(gsym), dim3(256), 0, st, ws, n, c, j);
-    unsigned gw = (unsigned)((L + 255) / 256);
-    hipLaunchKernelGGL(k_wcol, dim3(gw), dim3(256), 0, st, ws, n, c, j);
+    const int32_t gsym = (L + kSymvRows - 1) / kSymvRows, gw = (L + 255) / 256;
+    if (L >= kMergeBelow) {
+      hipLaunchKernelGGL(k_symv, dim3((unsigned)gsym), dim3(256), 0, st, ws, n, c, j);
+      hipLaunchKernelGGL(k_wcol, dim3((unsigned)gw), dim3(256), 0, st, ws, n, c, j);
+    } else {
+      hipLaunchKernelGGL(k_column, dim3((unsigned)(gw + gsym)), dim3(256), 0, st, ws, n, c, j, base,
+                         quiet_ticks(L, clock_khz));
+      base += gsym;
+    }
     // panel end: update the trailing matrix before the next panel's first symv
     if (j == kNB - 1 && c + 2 < n) {
       const int32_t k1 = c + 1;
@@ -1057,7 +1201,7 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
          o_ev = add(N),
          o_X = add((size_t)kMaxVec * N), o_lu = add((size_t)kMaxVec * 5 * N),
          o_Z = add(kNB * kMaxVec), o_T = add((size_t)npanel * kNB * kNB),
-         o_pw = add((size_t)2 * maxw * kPW), o_ps = add((size_t)maxblk);
+         o_pw = add((size_t)2 * maxw * kPW), o_ps = add((size_t)maxblk), o_sync = add(8);
   void *old = ctx->ework.p;
   EIG_TRY(eig_ensure(ctx, ctx->ework, need));
   char *base = (char *)ctx->ework.p;
@@ -1081,6 +1225,7 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
   ws.Tall = (double *)(base + o_T);
   ws.pw = (double *)(base + o_pw);
   ws.ps = (double *)(base + o_ps);
+  ws.sync = (int32_t *)(base + o_sync);
   ws.maxw = (int32_t)maxw;
   hipStream_t st = ctx->stream;
   if (lda != n) EIG_HIP(ctx, hipMemsetAsync(ws.a, 0, sizeof(double) * N * lda, st));
@@ -1095,6 +1240,8 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
   EIG_HIP(ctx, hipMemsetAsync(ws.xtmp, 0, sizeof(double) * lda, st));
 
   // ---- tridiagonalisation (graph replay for n above a few hundred)
+  int32_t clock_khz = 0;
+  EIG_HIP(ctx, hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, ctx->device));
   tbeg();
   const bool use_graph = !internal && n >= 256 && getenv("EIG_NO_GRAPH") == nullptr;
   if (use_graph) {
@@ -1104,7 +1251,7 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
       G.exec = nullptr;
       hipGraph_t graph;
       EIG_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      launch_tridiag(ws, n, st);
+      launch_tridiag(ws, n, clock_khz, st);
       EIG_HIP(ctx, hipStreamEndCapture(st, &graph));
       EIG_HIP(ctx, hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0));
       hipGraphDestroy(graph);
@@ -1113,7 +1260,7 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
     }
     EIG_HIP(ctx, hipGraphLaunch(G.exec, st));
   } else {
-    launch_tridiag(ws, n, st);
+    launch_tridiag(ws, n, clock_khz, st);
   }
   EIG_HIP(ctx, hipGetLastError());
   tend("tridiag");
@@ -1155,6 +1302,8 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
   }
   tend("eigvec");
   std::vector<double> asc(N);
+  int32_t hsync[2] = {0, 0};
+  EIG_HIP(ctx, hipMemcpyAsync(hsync, ws.sync, sizeof(hsync), hipMemcpyDeviceToHost, st));
   EIG_HIP(ctx, hipMemcpyAsync(asc.data(), ws.eval, sizeof(double) * N, hipMemcpyDeviceToHost, st));
   if (nvec > 0 && evecs)
     EIG_HIP(ctx, hipMemcpyAsync(evecs, ws.X, sizeof(double) * N * nvec, hipMemcpyDeviceToHost, st));
@@ -1165,6 +1314,8 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
                                   hipMemcpyDeviceToDevice, st));
   }
   EIG_HIP(ctx, hipStreamSynchronize(st));
+  if (hsync[1] != 0)
+    return eig_set_err(ctx, EIG_E_CONVERGE, "eigh: tridiagonalisation hand-off timed out");
   for (int64_t i = 0; i < N; ++i) {
     if (std::isnan(asc[i])) return eig_set_err(ctx, EIG_E_CONVERGE, "eigenvalue is NaN");
     if (evals) evals[i] = asc[N - 1 - i];
