@@ -321,8 +321,7 @@ __device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
   __shared__ double pqr[4][2 * kNB];  // raw panel dots per thread group
   __shared__ double pq[2 * kNB];      // p_q = V_q . v, q_q = W_q . v (zero for q >= j)
   __shared__ double cc[2 * kNB];      // V_q[c+1], W_q[c+1] (zero for q >= j)
-  __shared__ double tb[256][17];      // transpose buffer of the block reduction
-  __shared__ double tb2[16][16];
+  __shared__ double tb[256][17];      // per-wave transpose tiles of the panel-dot reduction
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int32_t c1 = c + 1;
   const int32_t lda = ws.lda;
@@ -393,17 +392,17 @@ __device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
     pq[comp] = q < j ? atc + scale * raw : 0.0;  // v = e_{c+1} + scale x beyond c+1
   }
   __syncthreads();
-  // alpha = -tau/2 w.v with w.v = tau (y.v - 2 sum_q p_q q_q)
-  double spq = 0.0;
-  for (int q = 0; q < j; ++q) spq += pq[q] * pq[kNB + q];
-  const double alpha = -0.5 * tau * (tau * (ydv - 2.0 * spq));
-  // w_pre at row c+1 (same in every block)
-  double wc1;
-  {
-    double y1 = yc1;
-    for (int q = 0; q < j; ++q) y1 -= cc[q] * pq[kNB + q] + cc[kNB + q] * pq[q];
-    wc1 = tau * y1;
+  // alpha = -tau/2 w.v with w.v = tau (y.v - 2 sum_q p_q q_q), and w_pre at row c+1
+  // (same in every block).  pq and cc are zero from q = j on, so the sums run over
+  // the whole panel (same bits as stopping at j) with every LDS read issued at once.
+  double spq = 0.0, y1 = yc1;
+#pragma unroll
+  for (int q = 0; q < kNB; ++q) {
+    spq += pq[q] * pq[kNB + q];
+    y1 -= cc[q] * pq[kNB + q] + cc[kNB + q] * pq[q];
   }
+  const double alpha = -0.5 * tau * (tau * (ydv - 2.0 * spq));
+  const double wc1 = tau * y1;
 #pragma unroll
   for (int q = 0; q < kNB; ++q) {
     yy -= vrow[q] * pq[kNB + q] + wrow[q] * pq[q];
@@ -443,35 +442,41 @@ __device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
       red[wv][2] = v1;
     }
   }
-  // components 0..31: V_q . x, 32..63: W_q . x, in chunks of 16 through LDS
+  // components 0..31: V_q . x, 32..63: W_q . x.  Each wave reduces its 64 rows,
+  // 16 components at a time, through its own rows of tb (a wave barrier, no block
+  // barrier): lane (part, k) sums 16 rows of component k, two exchanges add the
+  // parts, and lane 16 (k0 / 16) + k keeps component k0 + k.  One block barrier
+  // then joins the four waves.
+  {
+    double (*tw)[17] = tb + wv * 64;
+    const int k = lane & 15, part = lane >> 4;
+    double mine = 0.0;
 #pragma unroll
-  for (int k0 = 0; k0 < 2 * kNB; k0 += 16) {
-    const int q0 = k0 & (kNB - 1);
-    if (q0 < jn) {  // uniform
+    for (int k0 = 0; k0 < 2 * kNB; k0 += 16) {
+      const int q0 = k0 & (kNB - 1);
+      if (q0 < jn) {  // uniform
 #pragma unroll
-      for (int kk = 0; kk < 16; ++kk) tb[threadIdx.x][kk] = (k0 < kNB ? vrow[q0 + kk] : wrow[q0 + kk]) * xl;
-      __syncthreads();
-      {
-        const int k = threadIdx.x & 15, part = threadIdx.x >> 4;
+        for (int kk = 0; kk < 16; ++kk) tw[lane][kk] = (k0 < kNB ? vrow[q0 + kk] : wrow[q0 + kk]) * xl;
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
         double sum = 0.0;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) sum += tb[part * 16 + i][k];
-        tb2[part][k] = sum;
+        for (int i = 0; i < 16; ++i) sum += tw[part * 16 + i][k];
+        sum += __shfl_xor(sum, 16, 64);
+        sum += __shfl_xor(sum, 32, 64);
+        if (part == (k0 >> 4)) mine = sum;
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
       }
-      __syncthreads();
-      if (threadIdx.x < 16) {
-        const int q = q0 + threadIdx.x;
-        double sum = 0.0;
-#pragma unroll
-        for (int part = 0; part < 16; ++part) sum += tb2[part][threadIdx.x];
-        pwn[2 + k0 + threadIdx.x] = q < jn ? sum : 0.0;
-      }
-      __syncthreads();
-    } else if (threadIdx.x < 16) {
-      pwn[2 + k0 + threadIdx.x] = 0.0;
     }
+    pqr[wv][lane] = mine;
   }
   __syncthreads();
+  if (threadIdx.x < 2 * kNB) {
+    const int comp = threadIdx.x;
+    const double sum = (pqr[0][comp] + pqr[1][comp]) + (pqr[2][comp] + pqr[3][comp]);
+    pwn[2 + comp] = (comp & (kNB - 1)) < jn ? sum : 0.0;
+  }
   if (threadIdx.x == 0) {
     pwn[0] = (red[0][1] + red[1][1]) + (red[2][1] + red[3][1]);
     pwn[1] = (red[0][2] + red[1][2]) + (red[2][2] + red[3][2]);
