@@ -158,6 +158,26 @@ def test_eigh_clustered_and_structured(eng):
     assert np.allclose(a @ v.T, v.T * w[:4], atol=1e-8)
 
 
+def test_eigh_both_column_launch_shapes(eng):
+    """n = 2,305: columns with trailing order >= 2,000 run as two kernels (symv, then
+    w-column), shorter ones as one launch with the in-launch hand-off; the graph is
+    captured on the first call and replayed on the second, which must give the same
+    bits.  Bars: eigenvalues within 1e-10 * lambda_max of LAPACK (numpy's dsyevd),
+    residuals 1e-9 * lambda_max, orthonormal vectors."""
+    n = 2305
+    rng = np.random.default_rng(2305)
+    a = rng.standard_normal((n, n))
+    a = a @ a.T / n + np.diag(rng.uniform(0, 1, n))
+    _say(f"eigh n={n} (both launch shapes)")
+    w, v = eng.eigh(a, 10)
+    wr = np.linalg.eigvalsh(a)[::-1]
+    assert np.max(np.abs(w - wr)) <= 1e-10 * wr[0]
+    assert np.allclose(v @ v.T, np.eye(10), atol=1e-10)
+    assert np.max(np.abs(a @ v.T - v.T * w[:10])) <= 1e-9 * wr[0]
+    w2, v2 = eng.eigh(a, 10)
+    assert np.array_equal(w, w2) and np.array_equal(v, v2)
+
+
 def test_eigh_xtx_pipeline(eng):
     N, M = 200, 4000
     g = _rand_panel(21, M, N)
