@@ -278,7 +278,9 @@ __global__ __launch_bounds__(256, 2) void k_syrk_bf16(
     const int buf = gi & 1;
     // the next group (the last group loads itself again, unused)
     const int32_t gn = gi + 1 < ng ? gi + 1 : gi;
-    const uint32_t *th = tab[buf][0], *tl = tab[buf][1];
+    // this lane's half of the tables: the lane term joins the base once per group, so
+    // a lookup is one extract and one shift-add, the rest an immediate offset
+    const uint32_t *th = tab[buf][0] + hoff, *tl = tab[buf][1] + hoff;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       // the next group's codes and level entries go out once stage 1 has used the
@@ -295,7 +297,7 @@ __global__ __launch_bounds__(256, 2) void k_syrk_bf16(
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const uint32_t c = (word >> (16 * (s & 1) + 4 * q)) & 15u;
-          const uint32_t idx = (uint32_t)(((s * 4 + q) * 2) * 16) + hoff + c;
+          const uint32_t idx = (uint32_t)(((s * 4 + q) * 2) * 16) + c;
           hv[q] = th[idx];
           lw[q] = tl[idx];
         }
