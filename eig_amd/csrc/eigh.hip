@@ -782,28 +782,27 @@ __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec
 #pragma unroll
       for (int k = 0; k < kInvB; ++k) db[k] = (i0 + k + 1 < n) ? ws.d[i0 + k + 1] : 0.0;
     };
-    auto runf = [&](int32_t i0, const double (&eb)[kInvB + 1], const double (&db)[kInvB]) {
+    // full: the whole batch is inside the recurrence (no per-step bound check).  The
+    // pivot choice is a select, not a branch: lanes (vectors) pivot differently, and
+    // both reciprocals are formed, that of l off the serial chain.
+    auto runf = [&](bool full, int32_t i0, const double (&eb)[kInvB + 1],
+                    const double (&db)[kInvB]) {
 #pragma unroll
       for (int k = 0; k < kInvB; ++k) {
         const int32_t i = i0 + k;
-        if (i < n - 1) {
+        if (full || i < n - 1) {
           const double l = eb[k];
           const double an = db[k] - lam, bn = eb[k + 1];
-          if (fabs(acur) >= fabs(l)) {
-            double a0 = acur;
-            if (fabs(a0) < tiny) a0 = copysign(tiny, a0 == 0.0 ? 1.0 : a0);
-            const double r = rcp_nr(a0);
-            const double m = l * r;
-            u0[i] = r; u1[i] = bcur; u2[i] = 0.0; ml[i] = m; pv[i] = 0.0;
-            acur = an - m * bcur;
-            bcur = bn;
-          } else {
-            const double r = rcp_nr(l);  // l is loaded ahead: off the serial chain
-            const double m = acur * r;
-            u0[i] = r; u1[i] = an; u2[i] = bn; ml[i] = m; pv[i] = 1.0;
-            acur = bcur - m * an;
-            bcur = -m * bn;
-          }
+          const bool sw = fabs(acur) < fabs(l);  // row interchange
+          double a0 = acur;
+          if (fabs(a0) < tiny) a0 = copysign(tiny, a0 == 0.0 ? 1.0 : a0);
+          const double rl = rcp_nr(l), ra = rcp_nr(a0);
+          const double r = sw ? rl : ra;
+          const double m = (sw ? acur : l) * r;
+          u0[i] = r; u1[i] = sw ? an : bcur; u2[i] = sw ? bn : 0.0; ml[i] = m; pv[i] = sw ? 1.0 : 0.0;
+          const double na = sw ? bcur - m * an : an - m * bcur;
+          bcur = sw ? -m * bn : bn;
+          acur = na;
         }
       }
     };
@@ -812,9 +811,9 @@ __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec
       ldf(0, eA, dA);
       for (int32_t i0 = 0; i0 < n - 1; i0 += 2 * kInvB) {
         ldf(i0 + kInvB, eB, dB);
-        runf(i0, eA, dA);
+        if (i0 + kInvB <= n - 1) runf(true, i0, eA, dA); else runf(false, i0, eA, dA);
         ldf(i0 + 2 * kInvB, eA, dA);
-        runf(i0 + kInvB, eB, dB);
+        if (i0 + 2 * kInvB <= n - 1) runf(true, i0 + kInvB, eB, dB); else runf(false, i0 + kInvB, eB, dB);
       }
     }
     if (fabs(acur) < tiny) acur = copysign(tiny, acur == 0.0 ? 1.0 : acur);
@@ -850,18 +849,16 @@ __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec
           mb[k] = in ? ml[i0 + k] : 0.0;
         }
       };
-      auto runx = [&](int32_t i0, const double (&xb)[kInvB], const double (&pb)[kInvB],
+      auto runx = [&](bool full, int32_t i0, const double (&xb)[kInvB], const double (&pb)[kInvB],
                       const double (&mb)[kInvB]) {
 #pragma unroll
         for (int k = 0; k < kInvB; ++k) {
           const int32_t i = i0 + k;
-          if (i < n - 1) {
-            double nx = xb[k];
-            if (pb[k] != 0.0) {
-              const double t = carry; carry = nx; nx = t;
-            }
-            x[i] = carry;
-            carry = nx - mb[k] * carry;
+          if (full || i < n - 1) {
+            const bool sw = pb[k] != 0.0;
+            const double lo = sw ? xb[k] : carry, nx = sw ? carry : xb[k];
+            x[i] = lo;
+            carry = nx - mb[k] * lo;
           }
         }
       };
@@ -870,9 +867,10 @@ __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec
         ldx(0, xA, pA, mA);
         for (int32_t i0 = 0; i0 < n - 1; i0 += 2 * kInvB) {
           ldx(i0 + kInvB, xB, pB, mB);
-          runx(i0, xA, pA, mA);
+          if (i0 + kInvB <= n - 1) runx(true, i0, xA, pA, mA); else runx(false, i0, xA, pA, mA);
           ldx(i0 + 2 * kInvB, xA, pA, mA);
-          runx(i0 + kInvB, xB, pB, mB);
+          if (i0 + 2 * kInvB <= n - 1) runx(true, i0 + kInvB, xB, pB, mB);
+          else runx(false, i0 + kInvB, xB, pB, mB);
         }
       }
       x[n - 1] = carry;
@@ -891,12 +889,12 @@ __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec
           cb[k] = in ? u2[i] : 0.0;
         }
       };
-      auto runb = [&](int32_t i0, const double (&xb)[kInvB], const double (&ab)[kInvB],
+      auto runb = [&](bool full, int32_t i0, const double (&xb)[kInvB], const double (&ab)[kInvB],
                       const double (&bb)[kInvB], const double (&cb)[kInvB]) {
 #pragma unroll
         for (int k = 0; k < kInvB; ++k) {
           const int32_t i = i0 - k;
-          if (i >= 0) {
+          if (full || i >= 0) {
             const double xi = (xb[k] - bb[k] * x1 - cb[k] * x2) * ab[k];
             x[i] = xi;
             x2 = x1;
@@ -910,9 +908,10 @@ __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec
         ldb(n - 2, xA, aA, bA, cA);
         for (int32_t i0 = n - 2; i0 >= 0; i0 -= 2 * kInvB) {
           ldb(i0 - kInvB, xB, aB, bB, cB);
-          runb(i0, xA, aA, bA, cA);
+          if (i0 - kInvB + 1 >= 0) runb(true, i0, xA, aA, bA, cA); else runb(false, i0, xA, aA, bA, cA);
           ldb(i0 - 2 * kInvB, xA, aA, bA, cA);
-          runb(i0 - kInvB, xB, aB, bB, cB);
+          if (i0 - 2 * kInvB + 1 >= 0) runb(true, i0 - kInvB, xB, aB, bB, cB);
+          else runb(false, i0 - kInvB, xB, aB, bB, cB);
         }
       }
     }
