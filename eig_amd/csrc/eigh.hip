@@ -555,7 +555,7 @@ constexpr int kS2K = 2 * kNB;        // GEMM depth
 constexpr int kS2Ld = kS2K + 1;      // padded LDS row (conflict-free 16-row reads)
 // One block per 64-row x kS2Strip-tile strip: the strip's [V|W] rows are staged
 // once and reused for every 64x64 column tile, whose [W|V] rows are staged in turn.
-constexpr int kS2Strip = 4;
+constexpr int kS2Strip = 2;
 __global__ __launch_bounds__(256) void k_syr2k(EigWs ws, int32_t n, int32_t k0, int32_t nb) {
   __shared__ double sa[64][kS2Ld], sb[64][kS2Ld];
   const int32_t nt = (n - k0 + 63) / 64;
