@@ -182,35 +182,38 @@ __device__ inline void store_dev(double *p, double v) {
 }
 
 // ---------------------------------------------------------------- symv part
-constexpr int kSymvRows = 8;  // rows per block
+constexpr int kSymvRows = 8;      // symv rows per block in k_column
+constexpr int kSymvRowsLong = 4;  // in k_symv (long columns): 106 VGPRs, 4 waves/SIMD;
+                                  // 2 ms less than 8 rows at n = 5,000 (measured)
+template <int R>
 __device__ inline void symv_part(const EigWs &ws, int32_t n, int32_t c, int32_t j, int32_t blk) {
-  __shared__ double red[4][kSymvRows];
-  __shared__ double yv[kSymvRows];
+  __shared__ double red[4][R];
+  __shared__ double yv[R];
   __shared__ double sscale;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int32_t lda = ws.lda;
-  const int32_t r0 = c + 1 + blk * kSymvRows;
+  const int32_t r0 = c + 1 + blk * R;
   const int32_t a0 = (c + 1) & ~1;
   const int32_t npair = (((n + 1) & ~1) - a0) >> 1;
   const double2 *xp = (const double2 *)(ws.xtmp + a0);
   // rows past n read row r0 instead (in bounds, result discarded), so every load
   // in the loop is unconditional and all of them are in flight together
-  const double2 *rp[kSymvRows];
+  const double2 *rp[R];
 #pragma unroll
-  for (int rr = 0; rr < kSymvRows; ++rr)
+  for (int rr = 0; rr < R; ++rr)
     rp[rr] = (const double2 *)(ws.a + (int64_t)((r0 + rr) < n ? r0 + rr : r0) * lda + a0);
-  double acc[kSymvRows];
+  double acc[R];
 #pragma unroll
-  for (int rr = 0; rr < kSymvRows; ++rr) acc[rr] = 0.0;
+  for (int rr = 0; rr < R; ++rr) acc[rr] = 0.0;
   // the first pairs' loads go out before the reflector scalars are reduced
   int32_t p = threadIdx.x;
-  double2 x0 = make_double2(0.0, 0.0), x1 = x0, a0v[kSymvRows], a1v[kSymvRows];
+  double2 x0 = make_double2(0.0, 0.0), x1 = x0, a0v[R], a1v[R];
   const bool two = p + 256 < npair;
   if (two) {
     x0 = xp[p];
     x1 = xp[p + 256];
 #pragma unroll
-    for (int rr = 0; rr < kSymvRows; ++rr) {
+    for (int rr = 0; rr < R; ++rr) {
       a0v[rr] = rp[rr][p];
       a1v[rr] = rp[rr][p + 256];
     }
@@ -245,7 +248,7 @@ __device__ inline void symv_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
     vpair(x0, p, v0, v1);
     vpair(x1, p + 256, w0, w1);
 #pragma unroll
-    for (int rr = 0; rr < kSymvRows; ++rr)
+    for (int rr = 0; rr < R; ++rr)
       acc[rr] += (a0v[rr].x * v0 + a0v[rr].y * v1) + (a1v[rr].x * w0 + a1v[rr].y * w1);
     p += 512;
     have = p + 256 < npair;
@@ -253,7 +256,7 @@ __device__ inline void symv_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
       x0 = xp[p];
       x1 = xp[p + 256];
 #pragma unroll
-      for (int rr = 0; rr < kSymvRows; ++rr) {
+      for (int rr = 0; rr < R; ++rr) {
         a0v[rr] = rp[rr][p];
         a1v[rr] = rp[rr][p + 256];
       }
@@ -261,20 +264,20 @@ __device__ inline void symv_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
   }
   if (p < npair) {
     const double2 x0 = xp[p];
-    double2 a0v[kSymvRows];
+    double2 a0v[R];
 #pragma unroll
-    for (int rr = 0; rr < kSymvRows; ++rr) a0v[rr] = rp[rr][p];
+    for (int rr = 0; rr < R; ++rr) a0v[rr] = rp[rr][p];
     double v0, v1;
     vpair(x0, p, v0, v1);
 #pragma unroll
-    for (int rr = 0; rr < kSymvRows; ++rr) acc[rr] += a0v[rr].x * v0 + a0v[rr].y * v1;
+    for (int rr = 0; rr < R; ++rr) acc[rr] += a0v[rr].x * v0 + a0v[rr].y * v1;
   }
-  // butterfly transpose-reduce of the 8 row sums: after the exchanges at offsets 4,
-  // 2, 1 lane l holds row (l & 7) summed over its group of 8 lanes; three more
-  // exchanges sum the groups
-  static_assert(kSymvRows == 8, "the reduction below is written for 8 rows");
+  // butterfly transpose-reduce of the R row sums: after the exchanges at offsets
+  // R/2, ..., 1 lane l holds row (l & (R-1)) summed over its group of R lanes; the
+  // exchanges at R, 2R, ..., 32 sum the groups
+  static_assert(R == 4 || R == 8, "R rows per block: 4 or 8");
 #pragma unroll
-  for (int o = 4; o > 0; o >>= 1) {
+  for (int o = R / 2; o > 0; o >>= 1) {
     const bool b = (lane & o) != 0;
 #pragma unroll
     for (int k = 0; k < o; ++k) acc[k] = (b ? acc[k + o] : acc[k]) + __shfl_xor(b ? acc[k] : acc[k + o], o, 64);
@@ -282,12 +285,12 @@ __device__ inline void symv_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
   {
     double a = acc[0];
 #pragma unroll
-    for (int o = 8; o < 64; o <<= 1) a += __shfl_xor(a, o, 64);
-    if (lane < kSymvRows) red[wv][lane] = a;
+    for (int o = R; o < 64; o <<= 1) a += __shfl_xor(a, o, 64);
+    if (lane < R) red[wv][lane] = a;
   }
   __syncthreads();
   // per row: y, v into V[j] and A[c], and the row's y.v
-  if (threadIdx.x < kSymvRows) {
+  if (threadIdx.x < R) {
     const int rr = threadIdx.x;
     const int32_t r = r0 + rr;
     double val = 0.0;
@@ -305,7 +308,7 @@ __device__ inline void symv_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
   if (threadIdx.x == 0) {
     double sum = 0.0;
 #pragma unroll
-    for (int rr = 0; rr < kSymvRows; ++rr) sum += yv[rr];
+    for (int rr = 0; rr < R; ++rr) sum += yv[rr];
     store_dev(ws.ps + blk, sum);
   }
 }
@@ -316,7 +319,7 @@ __device__ inline void symv_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
 // but always meet a zero coefficient.
 template <class Wait>
 __device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t j, int32_t blk,
-                                 Wait &&wait) {
+                                 int32_t nps, Wait &&wait) {
   __shared__ double red[4][4];
   __shared__ double pqr[4][2 * kNB];  // raw panel dots per thread group
   __shared__ double pq[2 * kNB];      // p_q = V_q . v, q_q = W_q . v (zero for q >= j)
@@ -327,7 +330,6 @@ __device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
   const int32_t lda = ws.lda;
   const double *pwc = ws.pw + (int64_t)(c & 1) * ws.maxw * kPW;
   const int32_t nw = wcol_blocks(n, c);
-  const int32_t gsym = (n - c - 1 + kSymvRows - 1) / kSymvRows;
   // ---- loads of data from earlier launches go out before the hand-off wait: this
   // thread's row of the panel vectors (column j, written by this launch's symv part,
   // is rebuilt from x below) and of the trailing row c+1, its x, row c+1's panel
@@ -379,7 +381,7 @@ __device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
   // ---- the symv part's outputs: y and the per-block partials of y.v
   double yy = ws.y[rs];
   const double yc1 = ws.y[c1];
-  double yv = sum_strided(ws.ps, 1, threadIdx.x, 256, gsym);
+  double yv = sum_strided(ws.ps, 1, threadIdx.x, 256, nps);
   for (int o = 32; o > 0; o >>= 1) yv += __shfl_down(yv, o, 64);
   if (lane == 0) red[wv][0] = yv;
   pqr[wv][lane] = praw;
@@ -486,11 +488,12 @@ __device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
 // The two parts as kernels of their own, for long columns: there the symv part
 // streams a large trailing matrix and a launch boundary costs less than holding
 // w-column blocks and draining device-coherent stores inside one launch.
+template <int R>
 __global__ __launch_bounds__(256) void k_symv(EigWs ws, int32_t n, int32_t c, int32_t j) {
-  symv_part(ws, n, c, j, (int32_t)blockIdx.x);
+  symv_part<R>(ws, n, c, j, (int32_t)blockIdx.x);
 }
-__global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, int32_t j) {
-  wcol_part(ws, n, c, j, (int32_t)blockIdx.x, []() { return true; });
+__global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, int32_t j, int32_t nps) {
+  wcol_part(ws, n, c, j, (int32_t)blockIdx.x, nps, []() { return true; });
 }
 
 // ---------------------------------------------------------------- k_column
@@ -513,7 +516,7 @@ __global__ __launch_bounds__(256) void k_column(EigWs ws, int32_t n, int32_t c, 
   const int32_t gw = (n - c - 1 + 255) / 256;
   const int32_t gsym = (int32_t)gridDim.x - gw;
   if ((int32_t)blockIdx.x >= gw) {
-    symv_part(ws, n, c, j, (int32_t)blockIdx.x - gw);
+    symv_part<kSymvRows>(ws, n, c, j, (int32_t)blockIdx.x - gw);
     // each wave drains its own device-coherent stores, then one thread adds the ticket
     __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
@@ -544,7 +547,7 @@ __global__ __launch_bounds__(256) void k_column(EigWs ws, int32_t n, int32_t c, 
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     return true;
   };
-  wcol_part(ws, n, c, j, (int32_t)blockIdx.x, wait);
+  wcol_part(ws, n, c, j, (int32_t)blockIdx.x, gsym, wait);
 }
 
 // A[r][s] -= sum_q V[r][q] W[s][q] + W[r][q] V[s][q] for r, s >= k0: one FP64 MFMA
@@ -1156,8 +1159,9 @@ static void launch_tridiag(EigWs ws, int32_t n, int32_t clock_khz, hipStream_t s
     const int32_t L = n - c - 1;
     const int32_t gsym = (L + kSymvRows - 1) / kSymvRows, gw = (L + 255) / 256;
     if (L >= kMergeBelow) {
-      hipLaunchKernelGGL(k_symv, dim3((unsigned)gsym), dim3(256), 0, st, ws, n, c, j);
-      hipLaunchKernelGGL(k_wcol, dim3((unsigned)gw), dim3(256), 0, st, ws, n, c, j);
+      const int32_t g4 = (L + kSymvRowsLong - 1) / kSymvRowsLong;
+      hipLaunchKernelGGL(k_symv<kSymvRowsLong>, dim3((unsigned)g4), dim3(256), 0, st, ws, n, c, j);
+      hipLaunchKernelGGL(k_wcol, dim3((unsigned)gw), dim3(256), 0, st, ws, n, c, j, g4);
     } else {
       hipLaunchKernelGGL(k_column, dim3((unsigned)(gw + gsym)), dim3(256), 0, st, ws, n, c, j, base,
                          quiet_ticks(L, clock_khz));
@@ -1189,7 +1193,7 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
   auto tend = [&](const char *what) { if (!internal) eig_time_end(ctx, what); };
   const int64_t N = n;
   const int32_t lda = (int32_t)(((N + 7) / 8) * 8);
-  const int64_t maxblk = (N + kSymvRows - 1) / kSymvRows + 8;
+  const int64_t maxblk = (N + kSymvRowsLong - 1) / kSymvRowsLong + 8;  // symv partials
   const int64_t maxnv = (int64_t)kNB * kMaxVec;
   const int64_t maxw = (N + 255) / 256 + 2;
   const int64_t npanel = (N + kNB - 1) / kNB + 1;
