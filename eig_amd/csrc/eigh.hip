@@ -337,7 +337,7 @@ __device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
   const int32_t r = c1 + blk * 256 + threadIdx.x;
   const bool inr = r < n;
   const int32_t rs = inr ? r : c1;  // rows past n read row c+1 (in bounds, discarded)
-  // first rounds of the three partial sums: panel dots (component lane over blocks
+  // first batches of the three partial sums: panel dots (component lane over blocks
   // wv, wv+4, ...), and the norm and alpha of column c (blocks lane, lane+64, ...)
   double lp[8], ln0[8], ln1[8];
   load8(pwc + 2 + lane, kPW, wv, 4, nw, lp);
