@@ -281,6 +281,7 @@ def main():
     relerr3, cosmin3, allabs3 = errs(b3)
     flops = 4.0 * N * (N + 1) * M
     ach = flops / (b["syrk_ms"] * 1e-3) / 1e12
+    tri_bytes = 8 * (N - 1) * N * (2 * N - 1) // 6  # sum over L = 1 .. N-1 of L^2 doubles
     # HBM bytes per launch of the headline kernel, from the two rocprofv3 --pmc runs
     # (FETCH_SIZE, WRITE_SIZE) of this workload scaled by a 1 GiB copy
     traffic = None
@@ -315,6 +316,14 @@ def main():
                      "kernel": "k_syrk_bf16<true, true> (4 products)", "algorithmic_flops": flops},
         "fp64_syrk_roofline": {"achieved": round(N * (N + 1) * M / (f["syrk_ms"] * 1e-3) / 1e12, 2),
                                "peak": PEAK_FP64, "unit": "TFLOP/s"},
+        # the eigensolver's largest stage against HBM: its symv streams each column's
+        # L x L trailing square once (L = N-1 .. 1), timed as a whole stage (per-column
+        # latency included)
+        "tridiag_roofline": {"bound": "hbm",
+                             "achieved": round(tri_bytes / (b["parts"]["tridiag"] * 1e-3) / 1e9, 1),
+                             "peak": PEAK_HBM, "unit": "GB/s",
+                             "frac": round(tri_bytes / (b["parts"]["tridiag"] * 1e-3) / 1e9 / PEAK_HBM, 4),
+                             "algorithmic_bytes": tri_bytes, "ms": round(b["parts"]["tridiag"], 3)},
     }
     if not args.no_serve:
         log("serving leg")
