@@ -40,7 +40,7 @@ BF16_BLOCK = 256  # SNPs per FP32 accumulation block of the split-BF16 forms
 
 def one_pass(eng, xindex, form, nvec):
     eng.select(xindex)
-    eng.snpstats()
+    eng.snpstats(fetch=False)  # the statistics stay in HBM, where XTX reads them
     eng.xtx(form, BF16_BLOCK, want_matrix=False)
     ev, vec = eng.eigh_xtx(nvec)
     return ev, vec

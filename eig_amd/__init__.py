@@ -38,8 +38,8 @@ SIGNATURES = {
                                    C.c_uint32, C.c_uint64]),
     "eig_fetch_packed": (C.c_int, [_vp, C.c_int64, C.c_int64, _u8p]),
     "eig_select": (C.c_int, [_vp, _i32p, C.c_int32]),
-    "eig_snpstats": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _i32p,
-                               _i32p, _i32p, _f64p, _f64p, _i32p]),
+    "eig_snpstats": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _vp,
+                               _vp, _vp, _vp, _vp, _vp]),  # outputs may be NULL
     "eig_xtx": (C.c_int, [_vp, C.c_int32, C.c_int32, C.POINTER(C.c_double), _vp]),
     "eig_eigh_xtx": (C.c_int, [_vp, C.c_int32, _f64p, _vp]),
     "eig_eigh": (C.c_int, [_vp, _f64p, C.c_int32, C.c_int32, _f64p, _vp]),
@@ -179,15 +179,18 @@ class Engine:
         self._chk(self._L.eig_select(self.h, xi, len(xi)), "eig_select")
         self.nrows = len(xi)
 
-    def snpstats(self, live=None, altnorm=1, fancynorm=1, minallelecnt=1, maxmissing=9999999):
+    def snpstats(self, live=None, altnorm=1, fancynorm=1, minallelecnt=1, maxmissing=9999999,
+                 fetch=True):
+        """Column statistics on the device.  fetch=False leaves them there (the XTX,
+        loadings and lsqproject kernels read them in place) and returns None."""
         M = self.nsnp
-        out = dict(c0=np.zeros(M, np.int32), c1=np.zeros(M, np.int32),
-                   nmiss=np.zeros(M, np.int32), mean=np.zeros(M), fancy=np.zeros(M),
-                   status=np.zeros(M, np.int32))
+        keys = ("c0", "c1", "nmiss", "mean", "fancy", "status")
+        types = (np.int32, np.int32, np.int32, np.float64, np.float64, np.int32)
+        out = {k: np.zeros(M, t) for k, t in zip(keys, types)} if fetch else None
         lv = None if live is None else np.ascontiguousarray(live, dtype=np.uint8)
+        args = [_ptr(out[k]) if fetch else None for k in keys]
         self._chk(self._L.eig_snpstats(self.h, _ptr(lv), altnorm, fancynorm, minallelecnt,
-                                       maxmissing, out["c0"], out["c1"], out["nmiss"],
-                                       out["mean"], out["fancy"], out["status"]), "eig_snpstats")
+                                       maxmissing, *args), "eig_snpstats")
         return out
 
     def xtx(self, form=FORM_FP64, block=0, want_matrix=True):

@@ -82,6 +82,21 @@ def test_snpstats_live_mask(eng):
     assert np.array_equal(st["c0"][1::3], ref["c0"][1::3])
 
 
+def test_snpstats_without_fetch_feeds_xtx_the_same(eng):
+    """Statistics left on the device (no host outputs) give the same XTX bits as a
+    call that also copies them out."""
+    g = _rand_panel(12, 2000, 90)
+    rows = O.pack_genotypes(g)
+    eng.load_packed(rows, 90)
+    eng.select(np.arange(90, dtype=np.int32))
+    st = eng.snpstats()
+    _, m1 = eng.xtx(0)
+    assert eng.snpstats(fetch=False) is None
+    _, m2 = eng.xtx(0)
+    assert np.array_equal(m1, m2)
+    assert np.array_equal(st["c0"], O.snpstats(rows, np.arange(90, dtype=np.int32))["c0"])
+
+
 @pytest.mark.parametrize("N,M", [(100, 3000), (300, 4500), (129, 257)])
 def test_xtx_fp64_matches_oracle(eng, N, M):
     g = _rand_panel(N + M, M, N)
