@@ -503,12 +503,13 @@ __global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, in
 // ps stores (device-coherent stores) have completed.  Blocks 0 .. gw-1 run the
 // w-column part for one 256-row chunk each: they issue their loads of data from
 // earlier launches at once, then wait until all gsym tickets are in, then read y
-// and the y.v partials.  The waiters never hold up a symv block (there are at most
-// 182 of them, far fewer than the blocks the device holds at once), so the launch
-// cannot deadlock; the wait is bounded anyway, and a timeout sets sync[1] for the
-// host to report.  The waiters first sleep for quiet_ticks of the device's wall
-// clock (most of the symv part's expected duration), then poll.  The counter is cleared by k_col0 and grows monotonically over
-// the columns: column c's tickets are base .. base + gsym - 1.
+// and the y.v partials.  The waiters never hold up a symv block (k_column runs for
+// trailing orders below kMergeBelow, so there are at most 8 of them, far fewer than
+// the blocks the device holds at once), so the launch cannot deadlock; the wait is
+// bounded anyway, and a timeout sets sync[1] for the host to report.  The waiters
+// first sleep for quiet_ticks of the device's wall clock (most of the symv part's
+// expected duration), then poll.  The counter is cleared by k_col0 and grows
+// monotonically over the columns: column c's tickets are base .. base + gsym - 1.
 constexpr int64_t kHandoffSpin = 1 << 22;
 __global__ __launch_bounds__(256) void k_column(EigWs ws, int32_t n, int32_t c, int32_t j,
                                                 int32_t base, int64_t quiet_ticks) {
