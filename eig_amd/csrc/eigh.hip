@@ -120,8 +120,9 @@ __device__ inline double sum_strided(const double *__restrict__ p, int64_t strid
   return sum_rest(p, stride, start, step, end, v);
 }
 
+constexpr int kWRows = 64;  // rows per w-column block (four threads per row)
 __device__ inline int32_t wcol_blocks(int32_t n, int32_t c) {  // producer blocks of column c
-  return c == 0 ? 1 : (n - c + 255) / 256;
+  return c == 0 ? 1 : (n - c + kWRows - 1) / kWRows;
 }
 
 // dlarfg scalars of column c from the producers' partials, reduced by one wave in
@@ -321,34 +322,36 @@ template <class Wait>
 __device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t j, int32_t blk,
                                  int32_t nps, Wait &&wait) {
   __shared__ double red[4][4];
-  __shared__ double pqr[4][2 * kNB];  // raw panel dots per thread group
+  __shared__ double pqr[4][2 * kNB];  // raw panel dots per wave
   __shared__ double pq[2 * kNB];      // p_q = V_q . v, q_q = W_q . v (zero for q >= j)
   __shared__ double cc[2 * kNB];      // V_q[c+1], W_q[c+1] (zero for q >= j)
-  __shared__ double tb[256][17];      // per-wave transpose tiles of the panel-dot reduction
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int sub = threadIdx.x & 3;  // this thread's quarter of the panel: q = 8 sub .. 8 sub + 7
   const int32_t c1 = c + 1;
   const int32_t lda = ws.lda;
   const double *pwc = ws.pw + (int64_t)(c & 1) * ws.maxw * kPW;
   const int32_t nw = wcol_blocks(n, c);
   // ---- loads of data from earlier launches go out before the hand-off wait: this
-  // thread's row of the panel vectors (column j, written by this launch's symv part,
-  // is rebuilt from x below) and of the trailing row c+1, its x, row c+1's panel
-  // coefficients, and the panel-dot and norm partials
-  const int32_t r = c1 + blk * 256 + threadIdx.x;
+  // row's quarter of the panel vectors (column j, written by this launch's symv part,
+  // is rebuilt from x below) and its entry of the trailing row c+1, its x, row c+1's
+  // panel coefficients, and the panel-dot and norm partials
+  const int32_t r = c1 + blk * kWRows + (int32_t)(threadIdx.x >> 2);
   const bool inr = r < n;
   const int32_t rs = inr ? r : c1;  // rows past n read row c+1 (in bounds, discarded)
   // first batches of the three partial sums: panel dots (component lane over blocks
-  // wv, wv+4, ...), and the norm and alpha of column c (blocks lane, lane+64, ...)
-  double lp[8], ln0[8], ln1[8];
-  load8(pwc + 2 + lane, kPW, wv, 4, nw, lp);
+  // wv, wv+4, ..., three batches of eight), and the norm and alpha of column c
+  // (blocks lane, lane+64, ...)
+  double lp[3][8], ln0[8], ln1[8];
+#pragma unroll
+  for (int t = 0; t < 3; ++t) load8(pwc + 2 + lane, kPW, wv + 32 * t, 4, nw, lp[t]);
   load8(pwc, kPW, lane, 64, nw, ln0);
   load8(pwc + 1, kPW, lane, 64, nw, ln1);
-  double vrow[kNB], wrow[kNB];
+  double vrow[8], wrow[8];
   {
-    const double2 *vp = (const double2 *)(ws.V + (int64_t)rs * kNB);
-    const double2 *wp = (const double2 *)(ws.W + (int64_t)rs * kNB);
+    const double2 *vp = (const double2 *)(ws.V + (int64_t)rs * kNB + 8 * sub);
+    const double2 *wp = (const double2 *)(ws.W + (int64_t)rs * kNB + 8 * sub);
 #pragma unroll
-    for (int h = 0; h < kNB / 2; ++h) {
+    for (int h = 0; h < 4; ++h) {
       const double2 a2 = vp[h], b2 = wp[h];
       vrow[2 * h] = a2.x;
       vrow[2 * h + 1] = a2.y;
@@ -363,11 +366,15 @@ __device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
     const int q = threadIdx.x & (kNB - 1);
     atc = (threadIdx.x < kNB) ? ws.V[(int64_t)c1 * kNB + q] : ws.W[(int64_t)c1 * kNB + q];
   }
-  double praw;
+  double praw = 0.0;
   {
     const int comp = lane;  // 0..31 V dots, 32..63 W dots
     const int q = comp & (kNB - 1);
-    praw = sum_rest(pwc + 2 + comp, kPW, wv, 4, nw, lp);
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+      for (int u = 0; u < 8; ++u) praw += lp[t][u];
+    for (int32_t bb = wv + 96; bb < nw; bb += 4) praw += pwc[2 + comp + (int64_t)bb * kPW];
     praw = q < j ? praw : 0.0;
   }
   // every wave forms the scalars itself (same bits as the symv part's)
@@ -405,28 +412,40 @@ __device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
   }
   const double alpha = -0.5 * tau * (tau * (ydv - 2.0 * spq));
   const double wc1 = tau * y1;
+  // the row's panel corrections: each thread sums its quarter, two exchanges add the
+  // quarters (the same bits in all four threads of the row)
+  {
+    double sy = 0.0, sbse = 0.0;
 #pragma unroll
-  for (int q = 0; q < kNB; ++q) {
-    yy -= vrow[q] * pq[kNB + q] + wrow[q] * pq[q];
-    base -= cc[q] * wrow[q] + cc[kNB + q] * vrow[q];
+    for (int k = 0; k < 8; ++k) {
+      const int q = 8 * sub + k;
+      sy += vrow[k] * pq[kNB + q] + wrow[k] * pq[q];
+      sbse += cc[q] * wrow[k] + cc[kNB + q] * vrow[k];
+    }
+    sy += __shfl_xor(sy, 1, 64);
+    sbse += __shfl_xor(sbse, 1, 64);
+    sy += __shfl_xor(sy, 2, 64);
+    sbse += __shfl_xor(sbse, 2, 64);
+    yy -= sy;
+    base -= sbse;
   }
-  // this column's v_r, as the symv part formed it, joins vrow at j; the thread's
+  // this column's v_r, as the symv part formed it, joins vrow at j; the row's
   // W[r][j] joins wrow below
   const double vr = (r == c1) ? 1.0 : xr * scale;
 #pragma unroll
-  for (int q = 0; q < kNB; ++q) vrow[q] = (q == j) ? vr : vrow[q];
+  for (int k = 0; k < 8; ++k) vrow[k] = (8 * sub + k == j) ? vr : vrow[k];
   const double w = tau * yy;
   const double wj = w + alpha * vr;
   // row c+1 minus panel column j:  V[j][c1] (=1) W[j][r] + W[j][c1] V[j][r]
   const double x = base - w - wc1 * vr - 2.0 * alpha * vr;
   const bool live = inr && r >= c1 + 2;
-  if (inr) {
+  if (inr && sub == 0) {
     ws.W[(int64_t)r * kNB + j] = wj;
     ws.xtmp[r] = x;
     if (r == c1) ws.d[c1] = x;
   }
 #pragma unroll
-  for (int q = 0; q < kNB; ++q) wrow[q] = (q == j) ? wj : wrow[q];
+  for (int k = 0; k < 8; ++k) wrow[k] = (8 * sub + k == j) ? wj : wrow[k];
   // ---- this block's partials for column c+1, summed in a fixed order:
   // norm, the next alpha (row c+2, block 0), and if column c+1 continues the
   // panel, V_q . x and W_q . x for q <= j
@@ -434,7 +453,7 @@ __device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
   const double xl = live ? x : 0.0;
   double *pwn = ws.pw + (int64_t)(c1 & 1) * ws.maxw * kPW + (int64_t)blk * kPW;
   {
-    double v0 = xl * xl, v1 = (inr && r == c1 + 1) ? x : 0.0;
+    double v0 = sub == 0 ? xl * xl : 0.0, v1 = (sub == 0 && inr && r == c1 + 1) ? x : 0.0;
     for (int o = 32; o > 0; o >>= 1) {
       v0 += __shfl_down(v0, o, 64);
       v1 += __shfl_down(v1, o, 64);
@@ -444,34 +463,29 @@ __device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
       red[wv][2] = v1;
     }
   }
-  // components 0..31: V_q . x, 32..63: W_q . x.  Each wave reduces its 64 rows,
-  // 16 components at a time, through its own rows of tb (a wave barrier, no block
-  // barrier): lane (part, k) sums 16 rows of component k, two exchanges add the
-  // parts, and lane 16 (k0 / 16) + k keeps component k0 + k.  One block barrier
-  // then joins the four waves.
+  // panel dots V_q . x, W_q . x: a thread holds 16 products (its quarter of V, then
+  // of W).  Butterfly transpose-reduce over the wave's 16 rows (lane bits 5..2): the
+  // exchange at offset o keeps the half selected by bit o, so lane (i, sub) ends
+  // with product i (i = (lane >> 2) & 15) summed over the rows; then one pass over
+  // the four waves through LDS.
   {
-    double (*tw)[17] = tb + wv * 64;
-    const int k = lane & 15, part = lane >> 4;
-    double mine = 0.0;
+    double t[16];
 #pragma unroll
-    for (int k0 = 0; k0 < 2 * kNB; k0 += 16) {
-      const int q0 = k0 & (kNB - 1);
-      if (q0 < jn) {  // uniform
-#pragma unroll
-        for (int kk = 0; kk < 16; ++kk) tw[lane][kk] = (k0 < kNB ? vrow[q0 + kk] : wrow[q0 + kk]) * xl;
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        double sum = 0.0;
-#pragma unroll
-        for (int i = 0; i < 16; ++i) sum += tw[part * 16 + i][k];
-        sum += __shfl_xor(sum, 16, 64);
-        sum += __shfl_xor(sum, 32, 64);
-        if (part == (k0 >> 4)) mine = sum;
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-      }
+    for (int k = 0; k < 8; ++k) {
+      t[k] = vrow[k] * xl;
+      t[8 + k] = wrow[k] * xl;
     }
-    pqr[wv][lane] = mine;
+#pragma unroll
+    for (int off = 32; off >= 4; off >>= 1) {
+      const int half = off >> 2;
+      const bool bsel = (lane & off) != 0;
+#pragma unroll
+      for (int k = 0; k < half; ++k)
+        t[k] = (bsel ? t[k + half] : t[k]) + __shfl_xor(bsel ? t[k] : t[k + half], off, 64);
+    }
+    const int i = (lane >> 2) & 15;
+    const int comp = (i < 8) ? 8 * sub + i : kNB + 8 * sub + (i - 8);
+    pqr[wv][comp] = t[0];
   }
   __syncthreads();
   if (threadIdx.x < 2 * kNB) {
@@ -497,14 +511,14 @@ __global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, in
 }
 
 // ---------------------------------------------------------------- k_column
-// One launch per column with gw + gsym blocks (gw = ceil((n-c-1)/256) w-column
+// One launch per column with gw + gsym blocks (gw = ceil((n-c-1)/64) w-column
 // chunks, gsym = ceil((n-c-1)/8) symv row groups).  Blocks gw .. gw+gsym-1 run the
 // symv part for their 8 rows and then add one to a ticket counter once their y and
 // ps stores (device-coherent stores) have completed.  Blocks 0 .. gw-1 run the
-// w-column part for one 256-row chunk each: they issue their loads of data from
+// w-column part for one 64-row chunk each: they issue their loads of data from
 // earlier launches at once, then wait until all gsym tickets are in, then read y
 // and the y.v partials.  The waiters never hold up a symv block (k_column runs for
-// trailing orders below kMergeBelow, so there are at most 8 of them, far fewer than
+// trailing orders below kMergeBelow, so there are at most 32 of them, far fewer than
 // the blocks the device holds at once), so the launch cannot deadlock; the wait is
 // bounded anyway, and a timeout sets sync[1] for the host to report.  The waiters
 // first sleep for quiet_ticks of the device's wall clock (most of the symv part's
@@ -514,7 +528,7 @@ constexpr int64_t kHandoffSpin = 1 << 22;
 __global__ __launch_bounds__(256) void k_column(EigWs ws, int32_t n, int32_t c, int32_t j,
                                                 int32_t base, int64_t quiet_ticks) {
   __shared__ int32_t ok_sh;
-  const int32_t gw = (n - c - 1 + 255) / 256;
+  const int32_t gw = (n - c - 1 + kWRows - 1) / kWRows;
   const int32_t gsym = (int32_t)gridDim.x - gw;
   if ((int32_t)blockIdx.x >= gw) {
     symv_part<kSymvRows>(ws, n, c, j, (int32_t)blockIdx.x - gw);
@@ -1158,7 +1172,7 @@ static void launch_tridiag(EigWs ws, int32_t n, int32_t clock_khz, hipStream_t s
   for (int32_t c = 0; c + 1 < n; ++c) {
     const int32_t j = c % kNB;
     const int32_t L = n - c - 1;
-    const int32_t gsym = (L + kSymvRows - 1) / kSymvRows, gw = (L + 255) / 256;
+    const int32_t gsym = (L + kSymvRows - 1) / kSymvRows, gw = (L + kWRows - 1) / kWRows;
     if (L >= kMergeBelow) {
       const int32_t g4 = (L + kSymvRowsLong - 1) / kSymvRowsLong;
       hipLaunchKernelGGL(k_symv<kSymvRowsLong>, dim3((unsigned)g4), dim3(256), 0, st, ws, n, c, j);
@@ -1196,7 +1210,7 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
   const int32_t lda = (int32_t)(((N + 7) / 8) * 8);
   const int64_t maxblk = (N + kSymvRowsLong - 1) / kSymvRowsLong + 8;  // symv partials
   const int64_t maxnv = (int64_t)kNB * kMaxVec;
-  const int64_t maxw = (N + 255) / 256 + 2;
+  const int64_t maxw = (N + kWRows - 1) / kWRows + 2;
   const int64_t npanel = (N + kNB - 1) / kNB + 1;
   size_t need = 0;
   auto add = [&](size_t cnt) {
