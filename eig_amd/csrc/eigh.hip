@@ -12,7 +12,7 @@
 //              known up front from w.v = tau (y.v - 2 p.q); in the same pass the
 //              next row x = A[c+1] - (V W^T + W V^T)[c+1] and per-block partials
 //              of its norm and of the next column's panel dots V^T x, W^T x.
-//     Trailing orders of 2,000 and more run the parts as two kernels (k_symv,
+//     Trailing orders of 1,000 and more run the parts as two kernels (k_symv,
 //     k_wcol); shorter ones run them in one launch, k_column, whose w-column
 //     blocks load what they can up front and wait on a ticket counter for the
 //     symv blocks.
@@ -518,7 +518,7 @@ __global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, in
 // w-column part for one 64-row chunk each: they issue their loads of data from
 // earlier launches at once, then wait until all gsym tickets are in, then read y
 // and the y.v partials.  The waiters never hold up a symv block (k_column runs for
-// trailing orders below kMergeBelow, so there are at most 32 of them, far fewer than
+// trailing orders below kMergeBelow, so there are at most 16 of them, far fewer than
 // the blocks the device holds at once), so the launch cannot deadlock; the wait is
 // bounded anyway, and a timeout sets sync[1] for the host to report.  The waiters
 // first sleep for quiet_ticks of the device's wall clock (most of the symv part's
@@ -1163,9 +1163,10 @@ static int64_t quiet_ticks(int32_t L, int32_t clock_khz) {
 // Trailing order below which k_column runs (one launch per column); longer columns
 // run k_symv + k_wcol.  Measured on MI355X: merging at every length costs ~10 us per
 // column on 5,000-long columns (the w-column blocks held for the whole stream and the
-// store drain) and saves ~3 us per column on short ones; 2,000 was the best cut for
-// n = 3,000 and 5,000 (1,000 to 4,000 tried).
-constexpr int32_t kMergeBelow = 2000;
+// store drain) and saves a few us per column on short ones.  With the 64-row
+// w-column chunks, cuts of 500 to 1,000 were best for n = 3,000 and 5,000 (1 to
+// 3,000 tried; never merging costs 1.1 ms at n = 5,000).
+constexpr int32_t kMergeBelow = 1000;
 static void launch_tridiag(EigWs ws, int32_t n, int32_t clock_khz, hipStream_t st) {
   hipLaunchKernelGGL(k_col0, dim3(1), dim3(256), 0, st, ws, n);
   int32_t base = 0;  // hand-off tickets taken by the earlier columns

@@ -173,11 +173,11 @@ def test_eigh_clustered_and_structured(eng):
     assert np.allclose(a @ v.T, v.T * w[:4], atol=1e-8)
 
 
-@pytest.mark.parametrize("n", [2001, 2305])
+@pytest.mark.parametrize("n", [1001, 2305])
 def test_eigh_both_column_launch_shapes(eng, n):
-    """Columns with trailing order >= 2,000 run as two kernels (symv, then w-column),
-    shorter ones as one launch with the in-launch hand-off (n = 2,001: only the first
-    column is long; n = 2,305: 305 long columns, odd and even first rows).  The graph
+    """Columns with trailing order >= 1,000 run as two kernels (symv, then w-column),
+    shorter ones as one launch with the in-launch hand-off (n = 1,001: only the first
+    column is long; n = 2,305: 1,305 long columns, odd and even first rows).  The graph
     is captured on the first call and replayed on the second, which must give the
     same bits.  Bars: eigenvalues within 1e-10 * lambda_max of LAPACK (numpy's
     dsyevd), residuals 1e-9 * lambda_max, orthonormal vectors."""
