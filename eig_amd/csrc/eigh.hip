@@ -1018,7 +1018,8 @@ __global__ __launch_bounds__(256) void k_bt_prep(EigWs ws, int32_t n, int32_t la
 __global__ __launch_bounds__(256) void k_bt_step(EigWs ws, int32_t n, int32_t p, int32_t last,
                                                  int32_t nvec, int32_t apply_prev) {
   __shared__ double szbuf[kNB * kMaxVec];
-  __shared__ double sv[kNB][kBtRows + 1];
+  __shared__ double sv[kNB][kBtRows + 1];   // this panel's reflectors over the block's rows
+  __shared__ double svn[kNB][kBtRows + 1];  // the previous (higher) panel's
   __shared__ double sx[kMaxVec][kBtRows + 1];
   double(*sz)[kMaxVec] = (double(*)[kMaxVec])szbuf;
   const int32_t lda = ws.lda;
@@ -1026,6 +1027,10 @@ __global__ __launch_bounds__(256) void k_bt_step(EigWs ws, int32_t n, int32_t p,
   const int32_t nb = min(kNB, last - c0 + 1);
   const int32_t rbase = c0 + 1 + blockIdx.x * kBtRows;
   const int32_t rows = min(kBtRows, n - rbase);
+  const int32_t c0n = c0 + kNB;  // previous (higher) panel
+  const int32_t nbn = apply_prev ? min(kNB, last - c0n + 1) : 0;
+  // every operand goes to LDS in one round of loads: Z, the block's rows of X, and
+  // both panels' reflector entries over those rows
   if (apply_prev) {
     for (int idx = threadIdx.x; idx < kNB * nvec; idx += 256)
       sz[idx / nvec][idx % nvec] = ws.Z[(idx / nvec) * kMaxVec + (idx % nvec)];
@@ -1034,29 +1039,24 @@ __global__ __launch_bounds__(256) void k_bt_step(EigWs ws, int32_t n, int32_t p,
     const int o = idx / kBtRows, i = idx % kBtRows;
     sx[o][i] = (i < rows) ? ws.X[(int64_t)o * n + rbase + i] : 0.0;
   }
-  __syncthreads();
-  if (apply_prev) {
-    const int32_t c0n = c0 + kNB;  // previous (higher) panel
-    const int32_t nbn = min(kNB, last - c0n + 1);
-    for (int idx = threadIdx.x; idx < nvec * kBtRows; idx += 256) {
-      const int o = idx / kBtRows, i = idx % kBtRows;
-      const int32_t r = rbase + i;
-      if (i >= rows) continue;
-      double s = 0.0;
-      for (int q = 0; q < nbn; ++q) {
-        const int32_t cq = c0n + q;
-        if (r > cq) s += ws.a[(int64_t)cq * lda + r] * sz[q][o];
-      }
-      sx[o][i] -= s;
-      ws.X[(int64_t)o * n + r] = sx[o][i];
-    }
-  }
   for (int idx = threadIdx.x; idx < kNB * kBtRows; idx += 256) {
     const int q = idx / kBtRows, i = idx % kBtRows;
-    const int32_t r = rbase + i, cq = c0 + q;
+    const int32_t r = rbase + i, cq = c0 + q, cqn = c0n + q;
     sv[q][i] = (q < nb && i < rows && r > cq) ? ws.a[(int64_t)cq * lda + r] : 0.0;
+    svn[q][i] = (q < nbn && i < rows && r > cqn) ? ws.a[(int64_t)cqn * lda + r] : 0.0;
   }
-  __syncthreads();  // sz consumed, sv and sx complete
+  __syncthreads();
+  if (apply_prev) {
+    for (int idx = threadIdx.x; idx < nvec * kBtRows; idx += 256) {
+      const int o = idx / kBtRows, i = idx % kBtRows;
+      if (i >= rows) continue;
+      double s = 0.0;
+      for (int q = 0; q < nbn; ++q) s += svn[q][i] * sz[q][o];
+      sx[o][i] -= s;
+      ws.X[(int64_t)o * n + rbase + i] = sx[o][i];
+    }
+  }
+  __syncthreads();  // sx updated
   // this block's partial dots V_p^T X over its rows; k_bt_reduce sums them
   const int nd = kNB * nvec;
   for (int idx = threadIdx.x; idx < nd; idx += 256) {
