@@ -30,6 +30,7 @@ UNITS = {
     "project.hip": False,
     "serve.hip": False,
     "fpca.hip": False,
+    "gemm.hip": True,
 }
 
 COMMON = [
@@ -67,7 +68,7 @@ def build(verbose: bool = False) -> str:
                 print(out)
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs +
-         ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
+         ["-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     build_cli()
     return LIB

@@ -118,3 +118,11 @@ int eig_ensure_selg(eig_ctx *ctx);
 int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, double *evals,
                  double *evecs,
                  bool internal = false);
+
+// Column-major FP64 GEMM on MFMA with dgemm's conventions, batched by strides:
+// C_b = alpha op(A_b) op(B_b) + beta C_b (C is not read when beta == 0).  lower: only
+// the 64 x 64 tiles on or below the diagonal are formed (m == n).  Launched on `st`.
+int eig_gemm_f64(eig_ctx *ctx, hipStream_t st, bool ta, bool tb, int64_t m, int64_t n, int64_t k,
+                 double alpha, const double *A, int64_t lda, int64_t sA, const double *B,
+                 int64_t ldb, int64_t sB, double beta, double *C, int64_t ldc, int64_t sC,
+                 int32_t batch, bool lower = false);

@@ -31,6 +31,7 @@
 //     one-block kernel sums them in block order into Z_p = T_p V_p^T X.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "eig_internal.h"
@@ -1196,6 +1197,7 @@ struct TridiagGraph {
   int32_t n = -1;
   void *a = nullptr;
   hipGraphExec_t exec = nullptr;
+  std::mutex mu;  // every context on the device shares the cached graph
 };
 static TridiagGraph g_graph[16];  // per device
 
@@ -1270,6 +1272,7 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
   const bool use_graph = !internal && n >= 256 && getenv("EIG_NO_GRAPH") == nullptr;
   if (use_graph) {
     TridiagGraph &G = g_graph[ctx->device & 15];
+    std::lock_guard<std::mutex> lock(G.mu);
     if (G.exec == nullptr || G.n != n || G.a != (void *)ws.a || old != ctx->ework.p) {
       if (G.exec) hipGraphExecDestroy(G.exec);
       G.exec = nullptr;

@@ -17,11 +17,11 @@
 // split-K factor are picked by a timing tuner on the first call for a shape, and
 // the 2I+1 products of the power iteration are replayed as one captured graph.
 // The tall-skinny Gram and basis products are plain GEMMs (rocBLAS).
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "eig_internal.h"
@@ -676,11 +676,15 @@ struct FpcaTune {
 };
 
 struct FpcaState {
-  rocblas_handle blas = nullptr;
   hipGraphExec_t gexec = nullptr;
-  const void *gkey[4] = {nullptr, nullptr, nullptr, nullptr};
-  int64_t gshape[5] = {0, 0, 0, 0, 0};
+  // everything the captured power-iteration graph bakes in: buffer addresses (the
+  // workspace base and every offset in it), shape, form, launch shapes and the 1/ncols
+  // scale; a call whose signature differs re-captures
+  std::vector<int64_t> gsig;
   FpcaTune xa[kMaxCT + 1], xtb[kMaxCT + 1];
+  // held for the whole call: the graph, the tuners and the rocBLAS handle are shared by
+  // every context on the device
+  std::mutex mu;
 };
 static FpcaState g_fpca[16];
 
@@ -701,13 +705,6 @@ static FpcaState g_fpca[16];
       return eig_set_err(ctx, EIG_E_ARG, "fpca: column count out of range");                 \
   }
 
-#define FP_BLAS(ctx, call)                                                                   \
-  do {                                                                                       \
-    rocblas_status s_ = (call);                                                              \
-    if (s_ != rocblas_status_success)                                                        \
-      return eig_set_err((ctx), EIG_E_HIP, "%s:%d rocBLAS %s: %d", __FILE__, __LINE__, #call, \
-                         (int)s_);                                                           \
-  } while (0)
 
 namespace {
 struct Plan {
@@ -929,32 +926,24 @@ __global__ void k_sum_parts(const double *__restrict__ part, int32_t S, int32_t 
 }
 
 // C (m x n, column-major, ld ldc) = A B^T, A m x k (ld lda) and B n x k (ld ldb) column-major,
-// for a long k: k is cut into S equal chunks (one strided-batched product) and a
-// remainder, whose partial products are summed in chunk order.  A single product of
-// these shapes gives rocBLAS only a few output tiles to spread a long k over.
-static int gemm_nt_long(eig_ctx *ctx, rocblas_handle h, int32_t m, int32_t n, int64_t k,
-                        const double *A, int64_t lda, const double *B, int64_t ldb, double *C,
-                        int32_t ldc, double *part, size_t part_doubles, hipStream_t st) {
-  const double one = 1.0, zero = 0.0;
+// for a long k: k is cut into S equal chunks (one batched product) and a remainder,
+// whose partial products are summed in chunk order.  A single product of these shapes
+// has only a few output tiles to spread a long k over.
+static int gemm_nt_long(eig_ctx *ctx, int32_t m, int32_t n, int64_t k, const double *A,
+                        int64_t lda, const double *B, int64_t ldb, double *C, int32_t ldc,
+                        double *part, size_t part_doubles, hipStream_t st) {
   const int64_t mn = (int64_t)m * n;
   int64_t S = std::min<int64_t>(64, k / 4096);
   S = std::min<int64_t>(S, (int64_t)(part_doubles / std::max<int64_t>(mn, 1)) - 1);
-  if (S < 2) {
-    FP_BLAS(ctx, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, m, n,
-                               (rocblas_int)k, &one, A, (rocblas_int)lda, B, (rocblas_int)ldb,
-                               &zero, C, ldc));
-    return EIG_OK;
-  }
+  if (S < 2)
+    return eig_gemm_f64(ctx, st, false, true, m, n, k, 1.0, A, lda, 0, B, ldb, 0, 0.0, C, ldc, 0, 1);
   const int64_t kc = k / S, rem = k - S * kc;
-  FP_BLAS(ctx, rocblas_dgemm_strided_batched(
-                   h, rocblas_operation_none, rocblas_operation_transpose, m, n, (rocblas_int)kc,
-                   &one, A, (rocblas_int)lda, (rocblas_stride)(kc * lda), B, (rocblas_int)ldb,
-                   (rocblas_stride)(kc * ldb), &zero, part, m, (rocblas_stride)mn, (rocblas_int)S));
+  EIG_TRY(eig_gemm_f64(ctx, st, false, true, m, n, kc, 1.0, A, lda, kc * lda, B, ldb, kc * ldb, 0.0,
+                       part, m, mn, (int32_t)S));
   int32_t nparts = (int32_t)S;
   if (rem > 0) {
-    FP_BLAS(ctx, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, m, n,
-                               (rocblas_int)rem, &one, A + S * kc * lda, (rocblas_int)lda,
-                               B + S * kc * ldb, (rocblas_int)ldb, &zero, part + S * mn, m));
+    EIG_TRY(eig_gemm_f64(ctx, st, false, true, m, n, rem, 1.0, A + S * kc * lda, lda, 0,
+                         B + S * kc * ldb, ldb, 0, 0.0, part + S * mn, m, 0, 1));
     ++nparts;
   }
   hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, st,
@@ -1082,6 +1071,7 @@ extern "C" int eig_fpca_form(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint
   if (!ctx->have_stats) return eig_set_err(ctx, EIG_E_STATE, "eig_fpca before eig_snpstats");
   EIG_HIP(ctx, hipSetDevice(ctx->device));
   FpcaState &F = g_fpca[ctx->device & 15];
+  std::lock_guard<std::mutex> lock(F.mu);
   hipStream_t st = ctx->stream;
   const int64_t M = ctx->nsnp;
   const int32_t n = ctx->nrows, nst = ctx->nst, npad = nst * kTileS;
@@ -1136,11 +1126,6 @@ extern "C" int eig_fpca_form(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint
   uint16_t *tah = bf ? (uint16_t *)(base + o_ta) : nullptr, *tal = tah ? tah + (size_t)cxa * npad : nullptr;
   uint16_t *tbh = bf ? (uint16_t *)(base + o_tb) : nullptr, *tbl = tbh ? tbh + (size_t)cxb * Mpad : nullptr;
 
-  if (!F.blas) {
-    FP_BLAS(ctx, rocblas_create_handle(&F.blas));
-  }
-  FP_BLAS(ctx, rocblas_set_stream(F.blas, st));
-  FP_BLAS(ctx, rocblas_set_pointer_mode(F.blas, rocblas_pointer_mode_host));
 
   eig_time_begin(ctx);
   hipLaunchKernelGGL(k_fpca_levels, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, st,
@@ -1190,12 +1175,17 @@ extern "C" int eig_fpca_form(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint
                       1.0 / (double)ncols, st);
   };
   // ---- power iterations (captured once per shape and buffer set, then replayed)
-  const void *key[4] = {ctx->sel.p, lf, Q, part};
-  const int64_t shape[5] = {M, nst, L, I,
-                            bf ? -(int64_t)ctx->num_cu
-                               : (int64_t)F.xa[CTL].grid * 1000 + F.xtb[CTL].S};
-  const bool same = F.gexec && std::equal(key, key + 4, F.gkey) &&
-                    std::equal(shape, shape + 5, F.gshape) && old == ctx->scratch2.p;
+  double inv_ncols = 1.0 / (double)ncols;
+  int64_t inv_bits;
+  std::memcpy(&inv_bits, &inv_ncols, sizeof inv_bits);
+  const std::vector<int64_t> sig = {
+      (int64_t)(uintptr_t)ctx->sel.p, (int64_t)(uintptr_t)ctx->selg.p, (int64_t)(uintptr_t)base,
+      (int64_t)off, (int64_t)o_lf, (int64_t)o_a0, (int64_t)o_a1, (int64_t)o_q, (int64_t)o_part,
+      (int64_t)o_lbf, (int64_t)o_ta, (int64_t)o_tb, M, nst, K, L, I, (int64_t)bf, inv_bits,
+      bf ? (int64_t)bslots * 100000 + bxa_grid
+         : (int64_t)F.xa[CTL].grid * 1000 + F.xtb[CTL].S};
+  (void)old;
+  const bool same = F.gexec && F.gsig == sig;
   auto power = [&]() -> int {
     double *cur = A0, *nxt = A1;
     for (int32_t i = 0; i < I; ++i) {
@@ -1221,8 +1211,7 @@ extern "C" int eig_fpca_form(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint
       e = hipGraphInstantiate(&F.gexec, graph, nullptr, nullptr, 0);
       hipGraphDestroy(graph);
       if (e != hipSuccess) return eig_set_err(ctx, EIG_E_HIP, "fpca graph: %s", hipGetErrorString(e));
-      std::copy(key, key + 4, F.gkey);
-      std::copy(shape, shape + 5, F.gshape);
+      F.gsig = sig;
     }
     EIG_HIP(ctx, hipGraphLaunch(F.gexec, st));
   }
@@ -1235,22 +1224,20 @@ extern "C" int eig_fpca_form(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint
   // The blocks differ in scale by powers of the top eigenvalue, so one Gram matrix of
   // all of Q would lose the small directions that the SVD keeps.
   eig_time_begin(ctx);
-  const double one = 1.0, zero = 0.0, mone = -1.0;
   std::vector<double> w;
   int32_t r = 0;
   // W (nc columns, column-major nc x M at Wp, ld ldq) -= U_r (U_r^T W)
   auto project = [&](double *Wp, int32_t nc) -> int {
     if (r == 0) return EIG_OK;
-    EIG_TRY(gemm_nt_long(ctx, F.blas, r, nc, M, U, ldq, Wp, ldq, Cb, r, part, part_doubles, st));
-    FP_BLAS(ctx, rocblas_dgemm(F.blas, rocblas_operation_transpose, rocblas_operation_none, nc,
-                               (rocblas_int)M, r, &mone, Cb, r, U, (rocblas_int)ldq, &one, Wp,
-                               (rocblas_int)ldq));
+    EIG_TRY(gemm_nt_long(ctx, r, nc, M, U, ldq, Wp, ldq, Cb, r, part, part_doubles, st));
+    EIG_TRY(eig_gemm_f64(ctx, st, true, false, nc, M, r, -1.0, Cb, r, 0, U, ldq, 0, 1.0, Wp, ldq,
+                         0, 1));
     return EIG_OK;
   };
   // Out (keep columns) = orthonormal basis of the nc columns at Wp
   auto orth = [&](const double *Wp, int32_t nc, double *Out, int32_t &keep) -> int {
     keep = 0;
-    EIG_TRY(gemm_nt_long(ctx, F.blas, nc, nc, M, Wp, ldq, Wp, ldq, G, nc, part, part_doubles, st));
+    EIG_TRY(gemm_nt_long(ctx, nc, nc, M, Wp, ldq, Wp, ldq, G, nc, part, part_doubles, st));
     std::vector<double> g((size_t)nc * nc), dsc(nc);
     EIG_HIP(ctx, hipMemcpyAsync(g.data(), G, sizeof(double) * g.size(), hipMemcpyDeviceToHost, st));
     EIG_HIP(ctx, hipStreamSynchronize(st));
@@ -1270,9 +1257,8 @@ extern "C" int eig_fpca_form(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint
     for (int32_t k = 0; k < keep; ++k)
       for (int32_t i = 0; i < nc; ++i) v[(size_t)k * nc + i] /= sqrt(w[k]) * dsc[i];
     EIG_HIP(ctx, hipMemcpyAsync(V, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, st));
-    FP_BLAS(ctx, rocblas_dgemm(F.blas, rocblas_operation_transpose, rocblas_operation_none, keep,
-                               (rocblas_int)M, nc, &one, V, nc, Wp, (rocblas_int)ldq, &zero, Out,
-                               (rocblas_int)ldq));
+    EIG_TRY(eig_gemm_f64(ctx, st, true, false, keep, M, nc, 1.0, V, nc, 0, Wp, ldq, 0, 0.0, Out,
+                         ldq, 0, 1));
     return EIG_OK;
   };
   for (int32_t blk = 0; blk <= I; ++blk) {
@@ -1301,7 +1287,7 @@ extern "C" int eig_fpca_form(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint
   EIG_TRY(launch_xtb(ctx, P, CTr, F.xtb[CTr].grid, F.xtb[CTr].S, src, ldq, r, part, Bm,
                      (int32_t)ldq, 1.0, st));
   }
-  EIG_TRY(gemm_nt_long(ctx, F.blas, r, r, npad, Bm, ldq, Bm, ldq, G, r, part, part_doubles, st));
+  EIG_TRY(gemm_nt_long(ctx, r, r, npad, Bm, ldq, Bm, ldq, G, r, part, part_doubles, st));
   // top-K eigenpairs of B^T B (order r up to 256) by the FP64 eigensolver: all
   // eigenvalues, K eigenvectors into V (column-major r x K)
   {
@@ -1321,8 +1307,8 @@ extern "C" int eig_fpca_form(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint
   hipLaunchKernelGGL(k_scale_cols, dim3((unsigned)(((int64_t)r * K + 255) / 256)), dim3(256), 0,
                      st, V, r, K, (const double *)sm);
   // Ev (column-major K x npad == row-major npad x K) = V_K^T B^T
-  FP_BLAS(ctx, rocblas_dgemm(F.blas, rocblas_operation_transpose, rocblas_operation_none, K, npad,
-                             r, &one, V, r, Bm, (rocblas_int)ldq, &zero, Ev, K));
+  EIG_TRY(eig_gemm_f64(ctx, st, true, false, K, npad, r, 1.0, V, r, 0, Bm, ldq, 0, 0.0, Ev, K, 0,
+                       1));
   std::vector<double> ev((size_t)npad * K);
   EIG_HIP(ctx, hipMemcpyAsync(ev.data(), Ev, sizeof(double) * ev.size(), hipMemcpyDeviceToHost, st));
   EIG_HIP(ctx, hipStreamSynchronize(st));

@@ -138,6 +138,34 @@ static void read_settings(const Params *pp, Settings *s) {
   par_int(pp, "gpu:", &s->gpu);
 }
 
+/* Reference keys that change the SNP set, the samples, the normalisation or the
+ * outputs, and that this build does not implement (readcommands, smartpca.c:2055-2233;
+ * defaults smartpca.c:154-300).  A parfile that sets one of them to anything but the
+ * reference default stops with an error instead of producing a different PCA. */
+static void reject_unsupported(const Params *pp) {
+  static const char *const any_value[] = {
+      "badsnpname:", "xregionname:", "elllistname:", "topright:", "id2pops:",
+      "snpoutfilename:", "snpoutname:", "indoutfilename:", "indivoutname:",
+      "genooutfilename:", "genotypeoutname:", NULL};
+  static const char *const nonzero[] = {
+      "killr2:", "shrinkmode:", "newshrink:", "fastshrink:", "autoshrink:", "mpshrink:",
+      "missingmode:", "easymode:", "fstonly:", "qtmode:", "rounakmode:", NULL};
+  for (int i = 0; any_value[i]; ++i)
+    if (par_str(pp, any_value[i]))
+      die("parameter %s is not supported by this build", any_value[i]);
+  for (int i = 0; nonzero[i]; ++i) {
+    int v = 0;
+    par_int(pp, nonzero[i], &v);
+    if (v != 0) die("parameter %s is not supported by this build", nonzero[i]);
+  }
+  int v = -1;
+  par_int(pp, "popsizelimit:", &v);
+  if (v > 0) die("parameter popsizelimit: is not supported by this build");
+  double h = -1.0;
+  par_dbl(pp, "hwfilter:", &h);
+  if (h > 0.0) die("parameter hwfilter: is not supported by this build");
+}
+
 static double now_s(void) {
   struct timespec ts;
   clock_gettime(CLOCK_MONOTONIC, &ts);
@@ -148,7 +176,7 @@ static void log_deleted(const Settings *s, const char *id, const char *why) {
   if (!s->deletesnpoutname) return;
   FILE *f = fopen(s->deletesnpoutname, "a");
   if (!f) return;
-  fprintf(f, "%20s %s\n", id, why);
+  fprintf(f, "%-40s  %-40s\n", id, why); /* logdeletedsnp, mcio.c:4893-4900 */
   fclose(f);
 }
 
@@ -228,6 +256,7 @@ int main(int argc, char **argv) {
   Settings S;
   defaults(&S);
   read_settings(&pp, &S);
+  reject_unsupported(&pp);
   S.verbose = verbose;
   printf("parameter file: %s\n### THE INPUT PARAMETERS\n##PARAMETER NAME: VALUE\n", parname);
   for (int i = 0; i < pp.n; ++i) printf("%s %s\n", pp.key[i], pp.val[i]);
@@ -293,18 +322,15 @@ int main(int argc, char **argv) {
     }
   }
 
-  /* ---- SNP filters (smartpca.c:801-871) */
+  /* ---- SNP filters (smartpca.c:801-852): the chromosome tests first, then the
+     position tests, each logged on its own, also for SNPs already dropped */
   for (int64_t m = 0; m < M; ++m) {
     Snp *c = &snps[m];
-    if (c->ignore) continue;
     const char *why = NULL;
     if (S.noxdata && c->chrom == S.numchrom + 1) why = "chrom-X";
     else if (c->chrom == 0) why = "chrom-0";
     else if (c->chrom > S.numchrom + 1) why = "chrom-big";
     else if (c->chrom == S.zchrom) why = "chrom-deltet";
-    else if (S.xchrom > 0 && c->chrom != S.xchrom) why = "not-chrom";
-    else if (S.xchrom > 0 && lround(c->ppos) < S.lopos) why = "lopos";
-    else if (S.xchrom > 0 && lround(c->ppos) > S.hipos) why = "hipos";
     if (why) {
       c->ignore = 1;
       log_deleted(&S, c->id, why);
@@ -313,6 +339,20 @@ int main(int argc, char **argv) {
   /* nodata: <= 1 valid genotype among non-ignored axis individuals */
   int64_t nodata = 0;
   for (int64_t m = 0; m < M; ++m) {
+    Snp *c = &snps[m];
+    const long pos = lrint(c->ppos); /* nnint, nicksrc/vsubs.c:849-854 */
+    if (S.xchrom > 0 && c->chrom != S.xchrom) {
+      c->ignore = 1;
+      log_deleted(&S, c->id, "not-chrom");
+    }
+    if (S.xchrom > 0 && pos < S.lopos) {
+      c->ignore = 1;
+      log_deleted(&S, c->id, "lopos");
+    }
+    if (S.xchrom > 0 && pos > S.hipos) {
+      c->ignore = 1;
+      log_deleted(&S, c->id, "hipos");
+    }
     if (snps[m].ignore) continue;
     const uint8_t *row = rows + m * rlen;
     int nv = 0;
@@ -612,19 +652,39 @@ int main(int argc, char **argv) {
         fprintf(ofile, S.hiprec ? "%12.6f  " : "%10.4f  ", acoeffs[(int64_t)j * N + i]);
       fprintf(ofile, "%15s\n", inds[i].pop);
     }
-    /* snp weights (writesnpeigs, smartpca.c:3038-3104) */
+    /* snp weights (writesnpeigs, smartpca.c:3038-3104): loadings scaled to mean square
+       1 over the pass's SNPs, the 10 largest |weights| of each axis on stdout, then the
+       weights file */
+    for (int j = 0; j < K; ++j) {
+      double y = 0.0;
+      for (int64_t m = 0; m < Mk; ++m) y += ffvecs[(int64_t)j * Mk + m] * ffvecs[(int64_t)j * Mk + m];
+      double sc = sqrt((double)ncols / y);
+      for (int64_t m = 0; m < Mk; ++m) ffvecs[(int64_t)j * Mk + m] *= sc;
+    }
+    {
+      double *snpsc = malloc(sizeof(double) * (size_t)Mk);
+      for (int64_t m = 0; m < Mk; ++m) snpsc[m] = -99999.0;
+      for (int j = 0; j < K; ++j) {
+        for (int64_t m = 0; m < Mk; ++m)
+          if (!ign[m]) snpsc[m] = fabs(ffvecs[(int64_t)j * Mk + m]);
+        for (int k = 0; k < 10 && ncols > 10; ++k) {
+          int64_t kmax = 0; /* first maximum (vlmaxmin, nicksrc/vsubs.c:304-331) */
+          for (int64_t m = 1; m < Mk; ++m)
+            if (snpsc[m] > snpsc[kmax]) kmax = m;
+          if (snpsc[kmax] < 0) break;
+          printf("eigbestsnp %4d %20s %2d %12d %9.3f\n", j + 1, snps[kmax].id, snps[kmax].chrom,
+                 (int)lrint(snps[kmax].ppos), snpsc[kmax]);
+          snpsc[kmax] = -1.0;
+        }
+      }
+      free(snpsc);
+    }
     if (S.snpeigname) {
       FILE *f = fopen(S.snpeigname, "w");
       if (!f) die("can't open %s", S.snpeigname);
-      for (int j = 0; j < K; ++j) {
-        double y = 0.0;
-        for (int64_t m = 0; m < Mk; ++m) y += ffvecs[(int64_t)j * Mk + m] * ffvecs[(int64_t)j * Mk + m];
-        double sc = sqrt((double)ncols / y);
-        for (int64_t m = 0; m < Mk; ++m) ffvecs[(int64_t)j * Mk + m] *= sc;
-      }
       for (int64_t m = 0; m < Mk; ++m) {
         if (ign[m]) continue;
-        fprintf(f, "%20s %2d %12ld", snps[m].id, snps[m].chrom, lround(snps[m].ppos));
+        fprintf(f, "%20s %2d %12d", snps[m].id, snps[m].chrom, (int)lrint(snps[m].ppos));
         for (int j = 0; j < K; ++j)
           fprintf(f, S.hiprec ? " %12.6f" : " %9.3f", ffvecs[(int64_t)j * Mk + m]);
         fprintf(f, "\n");

@@ -73,6 +73,30 @@ def test_duplicate_parameter_is_fatal(tmp_path):
     assert r.returncode != 0 and "duplicate parameter" in r.stderr
 
 
+@pytest.mark.parametrize("key,val", [("killr2", "YES"), ("xregionname", "x.txt"),
+                                     ("shrinkmode", "YES"), ("hwfilter", "0.001"),
+                                     ("popsizelimit", "20")])
+def test_unimplemented_reference_parameter_is_fatal(tmp_path, key, val):
+    """A reference key that would change the PCA and that this build does not
+    implement stops the run before any input is read, instead of being ignored."""
+    p = tmp_path / "par"
+    p.write_text(f"genotypename: a\nsnpname: b\nindivname: c\n{key}: {val}\n")
+    r = run_cli(str(p), tmp_path, expect_ok=False)
+    assert r.returncode != 0 and f"{key}: is not supported" in r.stderr
+
+
+def test_reference_default_of_an_unimplemented_parameter_is_accepted(golden, tmp_path):
+    """Setting such a key to the reference's default is not an error: the run goes on
+    to read the inputs (here it stops at the deliberately truncated genotype file)."""
+    _copy_example(golden, tmp_path)
+    raw = (tmp_path / "packed.geno").read_bytes()
+    (tmp_path / "packed.geno").write_bytes(raw[:-20])
+    write_par(tmp_path / "par", genotypename="packed.geno", snpname="packed.snp",
+              indivname="packed.ind", killr2="NO", shrinkmode="NO", popsizelimit=-1)
+    r = run_cli(str(tmp_path / "par"), tmp_path, expect_ok=False)
+    assert r.returncode != 0 and "length mismatch" in r.stderr
+
+
 def test_geno_hash_mismatch_is_fatal(golden, tmp_path):
     _copy_example(golden, tmp_path)
     lines = (tmp_path / "packed.ind").read_text().splitlines()

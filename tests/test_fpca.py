@@ -148,3 +148,27 @@ def test_fpca_rejects_bad_shapes(eng):
         eng.fpca(4, 4, 2)          # L must exceed K
     with pytest.raises(eig_amd.EigError):
         eng.fpca(4, 30, 9)         # (I+1)*L > 256
+
+
+def test_fpca_split_bf16_calls_with_changing_k_and_mask(eng):
+    """Consecutive split-BF16 fastmode calls that change K (the size of the eigenvector
+    region of the workspace) and the SNP mask (the 1/ncols scale) at the same L and I
+    re-capture the power-iteration graph: each result matches the FP64 form."""
+    import eig_amd
+    g = _panel(31, 6000, 4, 60)
+    N = g.shape[1]
+    rows = O.pack_genotypes(g)
+    eng.load_packed(rows, N)
+    eng.select(np.arange(N, dtype=np.int32))
+    live = np.ones(6000, np.uint8)
+    for K, cut in ((10, 0), (3, 0), (3, 1500), (6, 0)):
+        live[:] = 1
+        live[:cut] = 0
+        eng.snpstats(live=live)
+        _say(f"fpca bf16x4 K={K} masked={cut}")
+        e64, v64 = eng.fpca(K, 12, 3, seed=5, form=eig_amd.FORM_FP64)
+        eb, vb = eng.fpca(K, 12, 3, seed=5, form=eig_amd.FORM_BF16X4)
+        # the three population axes (the bulk Ritz pairs of a 3-iteration run are not
+        # converged, and their form-to-form differences say nothing about the graph)
+        assert np.max(np.abs(eb[:3] - e64[:3]) / np.abs(e64[:3])) <= 1e-6, (K, cut)
+        assert np.max(np.abs(vb[:3] - v64[:3])) <= 1e-5, (K, cut)

@@ -247,3 +247,17 @@ def test_loadings_and_lsqproject_match_oracle(eng):
     ok = rs == 0
     assert np.allclose(a[:, ok], ra[:, ok], rtol=1e-9, atol=1e-12)
     assert np.allclose(b[:, ok], rb[:, ok], rtol=1e-9, atol=1e-12)
+
+
+def test_eigh_rejects_non_finite_input_before_any_launch(eng):
+    """eig_eigh checks the caller's matrix on the host: a NaN or Inf entry is an
+    argument error (EIG_E_ARG) and no kernel runs on it."""
+    import eig_amd
+    for bad in (np.nan, np.inf, -np.inf):
+        a = np.eye(6)
+        a[2, 4] = a[4, 2] = bad
+        _say(f"eigh non-finite {bad}")
+        with pytest.raises(eig_amd.EigError, match=r"rc=-1: .*not finite"):
+            eng.eigh(a, 2)
+    w, v = eng.eigh(np.diag([3.0, 2.0, 1.0]), 1)  # the context still works
+    assert np.allclose(w, [3.0, 2.0, 1.0]) and np.allclose(np.abs(v[0]), [1.0, 0.0, 0.0])
