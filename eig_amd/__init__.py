@@ -41,10 +41,15 @@ SIGNATURES = {
     "eig_snpstats": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _vp,
                                _vp, _vp, _vp, _vp, _vp]),  # outputs may be NULL
     "eig_xtx": (C.c_int, [_vp, C.c_int32, C.c_int32, C.POINTER(C.c_double), _vp]),
+    "eig_xtx_normal": (C.c_int, [_vp, _vp, C.c_int32, _vp, C.c_int32, _vp, _vp, _vp, C.c_int32,
+                                 C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32, _vp,
+                                 _vp, _vp, _vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                 _vp]),
     "eig_eigh_xtx": (C.c_int, [_vp, C.c_int32, _f64p, _vp]),
     "eig_eigh": (C.c_int, [_vp, _f64p, C.c_int32, C.c_int32, _f64p, _vp]),
     "eig_loadings": (C.c_int, [_vp, _f64p, C.c_int32, _vp, _vp, _vp]),
     "eig_lsqproject": (C.c_int, [_vp, _i32p, _f64p, _f64p, _i32p]),
+    "eig_lsqproject_fill": (C.c_int, [_vp, _i32p, _i32p, C.c_int32, _f64p, _f64p, _i32p]),
     "eig_fpca": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, _vp, _f64p, _vp]),
     "eig_fpca_form": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, _vp, C.c_int32,
                                 _f64p, _vp]),
@@ -199,6 +204,25 @@ class Engine:
         self._chk(self._L.eig_xtx(self.h, form, block, C.byref(y), _ptr(mat)), "eig_xtx")
         return y.value, mat
 
+    def xtx_normal(self, xtypes=None, npop=0, weight=None, ldregress=0, chrom=None, ppos=None,
+                   gpos=None, ldposlimit=1000000000, ldlimit=9999.0, ldr2lo=0.01, ldr2hi=0.95,
+                   minallelecnt=1, maxmissing=9999999, want_matrix=True):
+        """The non-lookup column path (usepopsformissing / weights / ldregress): returns
+        (trace_scale, ynumsnps, stats dict, matrix or None)."""
+        M = self.nsnp
+        st = {k: np.zeros(M, np.int32) for k in ("c0", "c1", "nmiss", "status")}
+        arr = lambda a, t: None if a is None else np.ascontiguousarray(a, dtype=t)
+        xt, w = arr(xtypes, np.int32), arr(weight, np.float64)
+        ch, pp, gp = arr(chrom, np.int32), arr(ppos, np.float64), arr(gpos, np.float64)
+        y, yn = C.c_double(0), C.c_double(0)
+        mat = np.zeros((self.nrows, self.nrows)) if want_matrix else None
+        self._chk(self._L.eig_xtx_normal(self.h, _ptr(xt), npop, _ptr(w), ldregress, _ptr(ch),
+                                         _ptr(pp), _ptr(gp), ldposlimit, ldlimit, ldr2lo, ldr2hi,
+                                         minallelecnt, maxmissing, _ptr(st["c0"]), _ptr(st["c1"]),
+                                         _ptr(st["nmiss"]), _ptr(st["status"]), C.byref(y),
+                                         C.byref(yn), _ptr(mat)), "eig_xtx_normal")
+        return y.value, yn.value, st, mat
+
     def eigh_xtx(self, nvec):
         ev = np.zeros(self.nrows)
         vec = np.zeros((nvec, self.nrows)) if nvec > 0 else None
@@ -239,6 +263,16 @@ class Engine:
         a = np.zeros((K, n)); b = np.zeros((K, n)); st = np.zeros(n, np.int32)
         self._chk(self._L.eig_lsqproject(self.h, use, a.reshape(-1), b.reshape(-1), st),
                   "eig_lsqproject")
+        return a, b, st
+
+
+    def lsqproject_fill(self, use, itype, npop, K):
+        use = np.ascontiguousarray(use, dtype=np.int32)
+        it = np.ascontiguousarray(itype, dtype=np.int32)
+        n = len(use)
+        a = np.zeros((K, n)); b = np.zeros((K, n)); st = np.zeros(n, np.int32)
+        self._chk(self._L.eig_lsqproject_fill(self.h, use, it, npop, a.reshape(-1), b.reshape(-1),
+                                              st), "eig_lsqproject_fill")
         return a, b, st
 
 

@@ -31,6 +31,7 @@ UNITS = {
     "serve.hip": False,
     "fpca.hip": False,
     "gemm.hip": True,
+    "normal.hip": False,
 }
 
 COMMON = [

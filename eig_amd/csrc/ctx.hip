@@ -86,7 +86,7 @@ extern "C" void eig_close(eig_ctx *ctx) {
                     &ctx->levbf,  &ctx->live,    &ctx->xtx,        &ctx->part,     &ctx->ework,
                     &ctx->evecs_dev, &ctx->ffvecs, &ctx->fxscal,   &ctx->projw,    &ctx->proj_out,
                     &ctx->use_dev, &ctx->ignore_dev, &ctx->scratch, &ctx->scratch2,
-                    &ctx->selg};
+                    &ctx->selg,   &ctx->fill};
   for (DevBuf *b : bufs) free_buf(*b);
   if (ctx->hostpin.p) hipHostFree(ctx->hostpin.p);
   hipEventDestroy(ctx->ev0);
@@ -441,6 +441,9 @@ extern "C" int eig_snpstats(eig_ctx *ctx, const uint8_t *live, int32_t altnorm,
     EIG_HIP(ctx, hipMemcpyAsync(ctx->live.p, live, M, hipMemcpyHostToDevice, ctx->stream));
     dlive = (const uint8_t *)ctx->live.p;
   }
+  ctx->have_live = live != nullptr;
+  ctx->altnorm = altnorm;
+  ctx->fancynorm = fancynorm;
   eig_time_begin(ctx);
   unsigned blocks = (unsigned)((M + 255) / 256);
   hipLaunchKernelGGL(k_snpstats, dim3(blocks), dim3(256), 0, ctx->stream,

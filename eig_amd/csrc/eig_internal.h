@@ -51,7 +51,11 @@ struct eig_ctx {
   DevBuf lev;    // double [nsnp][4] : levels of g = 0,1,2 and 0 for missing
   DevBuf levbf;  // uint32 [nsnp][4] : bf16 hi (low half) | bf16 lo (high half)
   DevBuf live;   // uint8 [nsnp]
+  bool have_live = false;          // live mask of the last eig_snpstats
+  int32_t altnorm = 1, fancynorm = 1;  // normalisation of the last eig_snpstats
   bool have_stats = false;
+  // lsqproject fill table of usepopsformissing: [nsnp][npop] doubles, then nind types
+  DevBuf fill;
 
   // XTX (normalised, row-major nrows x nrows) and partial sums
   DevBuf xtx, part;
@@ -126,3 +130,10 @@ int eig_gemm_f64(eig_ctx *ctx, hipStream_t st, bool ta, bool tb, int64_t m, int6
                  double alpha, const double *A, int64_t lda, int64_t sA, const double *B,
                  int64_t ldb, int64_t sB, double beta, double *C, int64_t ldc, int64_t sC,
                  int32_t batch, bool lower = false);
+
+// lsqproject's population fill table (usepopsformissing) from the loaded genotypes and
+// the current ignore mask: types itype[q] (nind host entries, < 0 = none), npop types,
+// use[q] the individuals projected, K the axes.  *fill = device table [nsnp][npop] (NaN:
+// no data), *d_itype = the types on the device.
+int eig_lsq_filltab(eig_ctx *ctx, const int32_t *use, const int32_t *itype, int32_t npop,
+                    int32_t K, double **fill, const int32_t **d_itype);

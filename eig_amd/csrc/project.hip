@@ -180,6 +180,8 @@ __global__ __launch_bounds__(64) void k_lsq_gram(const uint8_t *__restrict__ gen
                                                  const double *__restrict__ fancy,
                                                  const double *__restrict__ xmean,
                                                  const uint8_t *__restrict__ ignore,
+                                                 const double *__restrict__ fill,
+                                                 const int32_t *__restrict__ itype, int32_t npop,
                                                  double *__restrict__ partial, int32_t nblk) {
   __shared__ double se[64][kMaxK + 2];
   const int li = threadIdx.x;
@@ -193,7 +195,8 @@ __global__ __launch_bounds__(64) void k_lsq_gram(const uint8_t *__restrict__ gen
   for (int p = 0; p < nP; ++p) G[p] = 0.0;
 #pragma unroll
   for (int j = 0; j < K; ++j) r[j] = 0.0;
-  double kk = 0.0;
+  double kk = 0.0, nobs = 0.0;
+  const int32_t ty = (fill && q < nind) ? itype[q] : -1;
   for (int64_t mb = m0; mb < m1; mb += 64) {
     __syncthreads();
     {
@@ -212,8 +215,17 @@ __global__ __launch_bounds__(64) void k_lsq_gram(const uint8_t *__restrict__ gen
         if (fz < 0.0) continue;  // ignored SNP: reads as missing
         uint32_t b = geno[(mb + rr) * rlen + (q >> 2)];
         uint32_t code = (b >> ((3 - (q & 3)) << 1)) & 3u;
-        if (code == 3u) continue;
-        double x = (double)code * fz;
+        double gv = (double)code;
+        if (code == 3u) {
+          // usepopsformissing: the mean genotype of the individual's population type
+          // (lsqproj, smartpca.c:4680-4690); without one the SNP is unobserved
+          if (ty < 0) continue;
+          gv = fill[(mb + rr) * npop + ty];
+          if (isnan(gv)) continue;
+        } else {
+          nobs += 1.0;
+        }
+        double x = gv * fz;
         x = (x < -0.001) ? 0.0 : x - se[rr][K + 1];
         kk += 1.0;
         double e[K];
@@ -235,17 +247,21 @@ __global__ __launch_bounds__(64) void k_lsq_gram(const uint8_t *__restrict__ gen
 #pragma unroll
     for (int j = 0; j < K; ++j) out[nP + j] = r[j];
     out[nP + K] = kk;
+    out[nP + K + 1] = nobs;
   }
 }
 
 // reduce chunks in order, Cholesky solve (choldc/cholsl order of operations)
+// itype (may be NULL): with the fill, an individual of type >= 0 is also ignored when
+// its observed genotypes alone number <= K (lsqproj's first loop, smartpca.c:4637-4651).
 __global__ void k_lsq_solve(const double *__restrict__ partial, int64_t nchunk, int32_t nind,
-                            int32_t K, const int32_t *__restrict__ use, double *__restrict__ ans,
+                            int32_t K, const int32_t *__restrict__ use,
+                            const int32_t *__restrict__ itype, double *__restrict__ ans,
                             double *__restrict__ rhs_out, int32_t *__restrict__ st) {
   const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nind) return;
-  const int nP = K * (K + 1) / 2, W = nP + K + 1;
-  double acc[kMaxK * (kMaxK + 1) / 2 + kMaxK + 1];
+  const int nP = K * (K + 1) / 2, W = nP + K + 2;
+  double acc[kMaxK * (kMaxK + 1) / 2 + kMaxK + 2];
   for (int p = 0; p < W; ++p) acc[p] = 0.0;
   for (int64_t c = 0; c < nchunk; ++c) {
     const double *pp = partial + (c * (int64_t)nind + q) * W;
@@ -256,7 +272,7 @@ __global__ void k_lsq_solve(const double *__restrict__ partial, int64_t nchunk, 
     st[q] = 3;
     return;
   }
-  if (acc[nP + K] <= (double)K) {
+  if (acc[nP + K] <= (double)K || (itype && itype[q] >= 0 && acc[nP + K + 1] <= (double)K)) {
     st[q] = 1;
     return;
   }
@@ -365,15 +381,18 @@ extern "C" int eig_loadings(eig_ctx *ctx, const double *evecs, int32_t K, const 
   return EIG_OK;
 }
 
-extern "C" int eig_lsqproject(eig_ctx *ctx, const int32_t *use, double *acoeffs,
-                              double *bcoeffs, int32_t *status) {
+static int lsqproject(eig_ctx *ctx, const int32_t *use, const int32_t *itype, int32_t npop,
+                      double *acoeffs, double *bcoeffs, int32_t *status) {
   if (!ctx || !use) return eig_set_err(ctx, EIG_E_ARG, "eig_lsqproject: args");
   if (!ctx->have_load) return eig_set_err(ctx, EIG_E_STATE, "eig_lsqproject before eig_loadings");
   EIG_HIP(ctx, hipSetDevice(ctx->device));
   const int64_t M = ctx->nsnp;
   const int32_t nind = ctx->nind, K = ctx->last_K;
   hipStream_t st = ctx->stream;
-  const int nP = K * (K + 1) / 2, W = nP + K + 1;
+  const int nP = K * (K + 1) / 2, W = nP + K + 2;
+  double *d_fill = nullptr;
+  const int32_t *d_itype = nullptr;
+  if (itype) EIG_TRY(eig_lsq_filltab(ctx, use, itype, npop, K, &d_fill, &d_itype));
   const int64_t nchunk = (M + kLsqChunk - 1) / kLsqChunk;
   const int32_t nblk = (nind + 63) / 64;
   size_t part_b = 8 * (size_t)nchunk * nind * W;
@@ -390,11 +409,11 @@ extern "C" int eig_lsqproject(eig_ctx *ctx, const int32_t *use, double *acoeffs,
                                     0, st, (const uint8_t *)ctx->geno.p, ctx->rlen, M, nind,
                                     (const double *)ctx->ffvecs.p, d_fx,
                                     (const double *)ctx->fancy.p, d_xm,
-                                    (const uint8_t *)ctx->ignore_dev.p, (double *)ctx->scratch2.p,
-                                    nblk));
+                                    (const uint8_t *)ctx->ignore_dev.p, (const double *)d_fill,
+                                    d_itype, npop, (double *)ctx->scratch2.p, nblk));
   hipLaunchKernelGGL(k_lsq_solve, dim3((unsigned)((nind + 127) / 128)), dim3(128), 0, st,
                      (const double *)ctx->scratch2.p, nchunk, nind, K, (const int32_t *)d_use,
-                     d_ans, d_rhs, d_st);
+                     d_itype, d_ans, d_rhs, d_st);
   EIG_HIP(ctx, hipGetLastError());
   eig_time_end(ctx, "lsqproject");
   std::vector<double> ans((size_t)K * nind), rhs((size_t)K * nind);
@@ -418,4 +437,16 @@ extern "C" int eig_lsqproject(eig_ctx *ctx, const int32_t *use, double *acoeffs,
       for (int j = 0; j < K; ++j) bcoeffs[(size_t)j * nind + q] = rhs[(size_t)j * nind + q];
   }
   return EIG_OK;
+}
+
+extern "C" int eig_lsqproject(eig_ctx *ctx, const int32_t *use, double *acoeffs,
+                              double *bcoeffs, int32_t *status) {
+  return lsqproject(ctx, use, nullptr, 0, acoeffs, bcoeffs, status);
+}
+
+extern "C" int eig_lsqproject_fill(eig_ctx *ctx, const int32_t *use, const int32_t *itype,
+                                   int32_t npop, double *acoeffs, double *bcoeffs,
+                                   int32_t *status) {
+  if (!itype) return eig_set_err(ctx, EIG_E_ARG, "eig_lsqproject_fill: itype is required");
+  return lsqproject(ctx, use, itype, npop, acoeffs, bcoeffs, status);
 }

@@ -41,12 +41,13 @@ void die(const char *fmt, ...) {
 
 typedef struct {
   char *genotypename, *snpname, *indivname, *poplistname, *outputname, *outputvname;
-  char *outliername, *snpeigname, *grmoutname, *deletesnpoutname, *xtxform;
+  char *outliername, *snpeigname, *grmoutname, *deletesnpoutname, *xtxform, *weightname;
   int numeigs, numoutliter, numoutleigs, outliermode, altnormstyle, fancynorm;
   int minallelecnt, maxmissing, noxdata, numchrom, hiprec, regmode, hashcheck;
   int familynames, doproject, malexhet, grmbinary, xchrom, zchrom, lopos, hipos;
   int pordercheck, fastmode, fastdim, fastiter, bf16block, gpu, verbose;
-  double outlthresh;
+  int usepops, ldregress, ldposlimit;
+  double outlthresh, ldlimit, ldr2lo, ldr2hi;
   long seed;
 } Settings;
 
@@ -74,6 +75,10 @@ static void defaults(Settings *s) {
   s->fastdim = -1;
   s->fastiter = -1;
   s->bf16block = 256;
+  s->ldlimit = 9999.0;          /* smartpca.c:289-293 */
+  s->ldr2lo = 0.01;
+  s->ldr2hi = 0.95;
+  s->ldposlimit = 1000 * 1000 * 1000;
 }
 
 static void read_settings(const Params *pp, Settings *s) {
@@ -133,6 +138,15 @@ static void read_settings(const Params *pp, Settings *s) {
   par_int(pp, "outliermode:", &s->outliermode);
   par_int(pp, "numchrom:", &s->numchrom);
   par_int(pp, "pordercheck:", &s->pordercheck);
+  par_string(pp, "weightname:", &s->weightname);
+  par_int(pp, "usepopsformissing:", &s->usepops);
+  par_int(pp, "ldregress:", &s->ldregress);
+  par_int(pp, "nsnpldregress:", &s->ldregress);
+  par_dbl(pp, "ldlimit:", &s->ldlimit);
+  par_int(pp, "ldposlimit:", &s->ldposlimit);
+  par_dbl(pp, "ldr2lo:", &s->ldr2lo);
+  par_dbl(pp, "ldr2hi:", &s->ldr2hi);
+  par_dbl(pp, "maxdistldregress:", &s->ldlimit);
   par_string(pp, "xtxform:", &s->xtxform);
   par_int(pp, "bf16block:", &s->bf16block);
   par_int(pp, "gpu:", &s->gpu);
@@ -263,6 +277,10 @@ int main(int argc, char **argv) {
   if (!S.genotypename || !S.snpname || !S.indivname)
     die("genotypename, snpname and indivname are required");
   if (S.xchrom == S.numchrom + 1) S.noxdata = 0;
+  if (S.ldlimit <= 0 || S.ldposlimit <= 0) S.ldregress = 0; /* smartpca.c:743-744 */
+  /* usepopsformissing, ldregress and SNP weights take the non-lookup column path
+     (smartpca.c:995-1015) */
+  const int normal = S.usepops || S.ldregress > 0 || S.weightname != NULL;
   const int form = form_of(S.xtxform);
   if (S.deletesnpoutname) remove(S.deletesnpoutname);
 
@@ -395,6 +413,53 @@ int main(int argc, char **argv) {
       ++Mk;
     }
   if (Mk == 0) die("no data");
+  /* SNP weights (getweights, mcio.c:2298-2336): "ID weight" lines, default 1 */
+  double *weight = NULL;
+  if (S.weightname) {
+    weight = malloc(sizeof(double) * (size_t)Mk);
+    for (int64_t m = 0; m < Mk; ++m) weight[m] = 1.0;
+    FILE *f = fopen(S.weightname, "r");
+    if (!f) die("can't open %s", S.weightname);
+    char line[4096], id[4096], wv[4096];
+    while (fgets(line, sizeof line, f)) {
+      int nf = sscanf(line, "%4095s %4095s", id, wv);
+      if (nf < 1 || id[0] == '#' || !strcmp(id, "SNP_ID") || !strcmp(id, "Indiv_ID") ||
+          !strcmp(id, "Chr"))
+        continue;
+      if (nf < 2) continue;
+      for (int64_t m = 0; m < Mk; ++m)
+        if (!strcmp(snps[m].id, id)) {
+          weight[m] = atof(wv);
+          break;
+        }
+    }
+    fclose(f);
+  }
+  /* population index (in eglist) of every individual, -1 outside it */
+  int32_t *popidx = malloc(sizeof(int32_t) * N);
+  for (int32_t i = 0; i < N; ++i) {
+    popidx[i] = -1;
+    for (int k = 0; k < neg; ++k)
+      if (!strcmp(inds[i].pop, eglist[k])) { popidx[i] = k; break; }
+  }
+  /* xtypes as lsqproj reads it: indexed by individual number but filled by row
+     (smartpca.c:853, 1096-1100, 4664), -2 where no pass wrote it */
+  int32_t *xt_quirk = malloc(sizeof(int32_t) * N);
+  for (int32_t i = 0; i < N; ++i) xt_quirk[i] = -2;
+  int32_t *xtypes = malloc(sizeof(int32_t) * N);
+  int32_t *chromv = NULL;
+  double *pposv = NULL, *gposv = NULL;
+  if (S.ldregress > 0) {
+    chromv = malloc(sizeof(int32_t) * (size_t)Mk);
+    pposv = malloc(sizeof(double) * (size_t)Mk);
+    gposv = malloc(sizeof(double) * (size_t)Mk);
+    for (int64_t m = 0; m < Mk; ++m) {
+      chromv[m] = snps[m].chrom;
+      pposv[m] = snps[m].ppos;
+      gposv[m] = snps[m].gpos;
+    }
+  }
+  double ynumsnps = 0.0;
 
   /* ---- engine */
   eig_ctx *ctx = NULL;
@@ -488,9 +553,18 @@ int main(int argc, char **argv) {
       ncols += live[m];
     }
     double t0 = now_s();
+    for (int32_t r = 0; r < nrows; ++r) xt_quirk[r] = xtypes[r] = popidx[xindex[r]];
     CHK(eig_select(ctx, xindex, nrows));
     CHK(eig_snpstats(ctx, live, S.altnormstyle, S.fancynorm, S.minallelecnt, S.maxmissing, c0,
                      c1, nmiss, mean, fancy, status));
+    double yscale = 0.0;
+    int rc;
+    if (normal)
+      rc = eig_xtx_normal(ctx, S.usepops ? xtypes : NULL, neg, weight, S.ldregress, chromv, pposv,
+                          gposv, S.ldposlimit, S.ldlimit, S.ldr2lo, S.ldr2hi, S.minallelecnt,
+                          S.maxmissing, c0, c1, nmiss, status, &yscale, &ynumsnps, NULL);
+    else
+      rc = eig_xtx(ctx, form, S.bf16block, &yscale, NULL);
     int nkill = 0;
     nused = 0;
     for (int64_t m = 0; m < Mk; ++m) {
@@ -509,8 +583,7 @@ int main(int argc, char **argv) {
       }
     }
     printf("number of samples used: %d number of snps used: %lld\n", nrows, (long long)nused);
-    double yscale = 0.0;
-    int rc = eig_xtx(ctx, form, S.bf16block, &yscale, NULL);
+    if (!normal) ynumsnps = (double)nused;
     if (rc == EIG_E_NUMERIC) die("%s", eig_last_error(ctx));
     CHK(rc);
     double t1 = now_s();
@@ -548,8 +621,15 @@ int main(int argc, char **argv) {
   /* ---- GRM (dumpgrm, smartpca.c:3687-3808) */
   if (S.grmoutname) {
     double *xtx = malloc(sizeof(double) * (size_t)nrows * nrows);
-    double ys;
-    CHK(eig_xtx(ctx, form, S.bf16block, &ys, xtx));
+    double ys, yn;
+    if (normal)
+      CHK(eig_xtx_normal(ctx, S.usepops ? xtypes : NULL, neg, weight, S.ldregress, chromv, pposv,
+                         gposv, S.ldposlimit, S.ldlimit, S.ldr2lo, S.ldr2hi, S.minallelecnt,
+                         S.maxmissing, NULL, NULL, NULL, NULL, &ys, &yn, xtx));
+    else
+      CHK(eig_xtx(ctx, form, S.bf16block, &ys, xtx));
+    /* dumpgrm takes ynumsnps as an int (smartpca.c:1278, 3770) */
+    const int grm_nsnp = (int)ynumsnps;
     char nm[4096];
     snprintf(nm, sizeof nm, "%s.id", S.grmoutname);
     FILE *f = fopen(nm, "w");
@@ -562,7 +642,7 @@ int main(int argc, char **argv) {
       snprintf(nm, sizeof nm, "%s.N.bin", S.grmoutname);
       f = fopen(nm, "wb");
       if (!f) die("can't open %s", nm);
-      int32_t w = (int32_t)nused;
+      int32_t w = (int32_t)grm_nsnp;
       for (int64_t a = 0; a < (int64_t)nrows * (nrows + 1) / 2; ++a) fwrite(&w, 4, 1, f);
       fclose(f);
       snprintf(nm, sizeof nm, "%s.bin", S.grmoutname);
@@ -581,7 +661,7 @@ int main(int argc, char **argv) {
       double rec = (double)nrows / tr;
       for (int32_t a = 0; a < nrows; ++a)
         for (int32_t b = 0; b <= a; ++b)
-          fprintf(f, "%d %d %lld %0.6f\n", a + 1, b + 1, (long long)nused,
+          fprintf(f, "%d %d %d %0.6f\n", a + 1, b + 1, grm_nsnp,
                   xtx[(int64_t)a * nrows + b] * rec);
       fclose(f);
     }
@@ -623,7 +703,10 @@ int main(int argc, char **argv) {
     if (S.regmode) {
       int32_t *use = malloc(4 * (size_t)N), *lst = malloc(4 * (size_t)N);
       for (int32_t i = 0; i < N; ++i) use[i] = !inds[i].ignore;
-      CHK(eig_lsqproject(ctx, use, acoeffs, bcoeffs, lst));
+      if (S.usepops)
+        CHK(eig_lsqproject_fill(ctx, use, xt_quirk, neg, acoeffs, bcoeffs, lst));
+      else
+        CHK(eig_lsqproject(ctx, use, acoeffs, bcoeffs, lst));
       for (int32_t i = 0; i < N; ++i)
         if (lst[i] == 1) {
           inds[i].ignore = 1;

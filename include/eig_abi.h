@@ -105,6 +105,35 @@ int eig_snpstats(eig_ctx *ctx, const uint8_t *live, int32_t altnorm, int32_t fan
 int eig_xtx(eig_ctx *ctx, int32_t form, int32_t bf16_block, double *trace_scale,
             double *xtx_out);
 
+/* The non-lookup column path: XTX from explicit FP64 columns, for the reference's
+ * usepopsformissing, weightname and ldregress options (getcolxz + fvadjust,
+ * smartpca.c:3129-3216, 2236-2279; weights 1150-1152; ldregx + bumpldvv 2691-2826,
+ * 1155-1170; domult_increment_normal 3498-3561; symit2 + trace scaling 480-508,
+ * 1222-1236).  Call after eig_snpstats (its live mask and normalisation are reused);
+ * replaces that call's statistics with this path's:
+ *   xtypes (nrows, may be NULL): population index in [0, npop) of each selected
+ *     sample -- a missing genotype takes its population's mean genotype at that SNP
+ *     when the population has one (usepopsformissing: YES);
+ *   weight (nsnp, may be NULL): each SNP's column is scaled by its weight;
+ *   ldregress > 0: each column is replaced by its least-squares residual on the
+ *     preceding kept SNPs of its chromosome in the reference's buffer of at most
+ *     ldregress SNPs within ldposlimit bases (ppos) and ldlimit Morgans (gpos),
+ *     using the sources whose squared correlation exceeds ldr2lo; a SNP whose squared
+ *     correlation with one of them exceeds ldr2hi contributes nothing.  chrom, ppos,
+ *     gpos: nsnp entries (required when ldregress > 0).
+ * The keep rule counts missing genotypes after the fill.  c0/c1/nmiss/status (nsnp,
+ * each may be NULL) as eig_snpstats; the per-SNP mean and normalisation kept on the
+ * device (used by eig_loadings and eig_lsqproject) are the filled column's, the levels
+ * used by eig_loadings the plain column's (getcolxf, smartpca.c:3564-3595).
+ * *ynumsnps: the GRM's SNP count (kept SNPs, or with ldregress the sum of residual to
+ * column squared norms, smartpca.c:1157-1162).  Always FP64. */
+int eig_xtx_normal(eig_ctx *ctx, const int32_t *xtypes, int32_t npop, const double *weight,
+                   int32_t ldregress, const int32_t *chrom, const double *ppos, const double *gpos,
+                   int32_t ldposlimit, double ldlimit, double ldr2lo, double ldr2hi,
+                   int32_t minallelecnt, int32_t maxmissing, int32_t *c0, int32_t *c1,
+                   int32_t *nmiss, int32_t *status, double *trace_scale, double *ynumsnps,
+                   double *xtx_out);
+
 /* Eigen-decomposition of the device-resident normalised XTX: eigvecs
  * (eigsubs.c:39-55) without its LAPACK call.  evals (nrows, descending) get every
  * eigenvalue; evecs (nvec x nrows, row j = eigenvector j, unit norm) the top
@@ -131,6 +160,15 @@ int eig_loadings(eig_ctx *ctx, const double *evecs, int32_t K, const uint8_t *ig
  * failed (previous individual's answer kept, as the reference does), 3 not used. */
 int eig_lsqproject(eig_ctx *ctx, const int32_t *use, double *acoeffs, double *bcoeffs,
                    int32_t *status);
+
+/* eig_lsqproject with usepopsformissing's fill (lsqproj, smartpca.c:4655-4745): itype[q]
+ * (nind) is individual q's population type (< 0: none).  A missing genotype of an
+ * individual with a type takes the mean genotype of that type at the SNP, over the
+ * individuals in use with that type (those with <= K observed genotypes excluded),
+ * when there is one; such an individual is also ignored (status 1) when its observed
+ * genotypes alone number <= K. */
+int eig_lsqproject_fill(eig_ctx *ctx, const int32_t *use, const int32_t *itype, int32_t npop,
+                        double *acoeffs, double *bcoeffs, int32_t *status);
 
 /* fastmode: randomized block-Krylov PCA (kjg_fpca, ksrc/kjg_fpca.c:24-101) of the
  * selected samples over every SNP not marked dead in the last eig_snpstats (the

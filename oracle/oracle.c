@@ -512,3 +512,88 @@ void orc_eigscale(const double *acoeffs, const double *bcoeffs, int32_t numindiv
         eigscale[j] = ab / aa;
     }
 }
+
+/* ---- lsqproject with usepopsformissing's fill: smartpca.c:4606-4757 ----------------
+ * itype[q]: individual q's population type (< 0: none).  First loop (4637-4651): an
+ * individual in use with a type and <= K observed genotypes over the SNPs not ignored
+ * is ignored (status 1).  Fill table (4655-4678): per type and SNP, the sum and count
+ * of observed genotypes over the individuals still in use with a type.  Second loop:
+ * a missing genotype of an individual with a type takes sum/count when count > 0.1,
+ * then fixxrow; the regression runs over the SNPs whose (filled) genotype is >= 0. */
+void orc_lsqproj_fill(const uint8_t *rows, int64_t rlen, int64_t ncols, int32_t numindivs,
+                      const int32_t *use, const int32_t *itype, int32_t npop,
+                      const int32_t *ignore, const double *xmean, const double *xfancy,
+                      const double *ffvecs, const double *fxscal, int32_t K, double *acoeffs,
+                      double *bcoeffs, int32_t *status)
+{
+    double *raw = malloc(sizeof(double) * (size_t)ncols);
+    double *xr = malloc(sizeof(double) * (size_t)ncols);
+    double *xnum = calloc((size_t)npop * ncols, sizeof(double));
+    double *xden = calloc((size_t)npop * ncols, sizeof(double));
+    double regans[64];
+    for (int j = 0; j < K; ++j) regans[j] = 0.0;
+    for (int32_t q = 0; q < numindivs; ++q) {
+        status[q] = use[q] ? 0 : 3;
+        if (!use[q] || itype[q] < 0) continue;
+        fix_row(rows, rlen, ncols, q, ignore, xmean, xfancy, raw, xr);
+        int64_t x = 0;
+        for (int64_t m = 0; m < ncols; ++m) x += raw[m] >= 0.0;
+        if (x <= K) status[q] = 1;
+    }
+    for (int32_t q = 0; q < numindivs; ++q) {
+        if (status[q] != 0 || itype[q] < 0) continue;
+        fix_row(rows, rlen, ncols, q, ignore, xmean, xfancy, raw, xr);
+        const int32_t t = itype[q];
+        for (int64_t m = 0; m < ncols; ++m)
+            if (raw[m] >= 0.0) { xnum[t * ncols + m] += raw[m]; xden[t * ncols + m] += 1.0; }
+    }
+    for (int pass = 0; pass < 2; ++pass)
+        for (int32_t q = 0; q < numindivs; ++q) {
+            if (status[q] == 1 || status[q] == 3) continue;
+            fix_row(rows, rlen, ncols, q, ignore, xmean, xfancy, raw, xr);
+            const int32_t t = itype[q];
+            if (t >= 0)
+                for (int64_t m = 0; m < ncols; ++m) {
+                    const int64_t u = t * ncols + m;
+                    if (raw[m] < 0.0 && xden[u] > 0.1) raw[m] = xnum[u] / xden[u];
+                }
+            for (int64_t m = 0; m < ncols; ++m) {
+                double y = raw[m] * xfancy[m];
+                xr[m] = (y < -0.001) ? 0.0 : y - xmean[m];
+            }
+            if (pass == 1) { /* bcoeffs */
+                for (int j = 0; j < K; ++j) {
+                    double acc = 0.0;
+                    const double *fp = ffvecs + (int64_t)j * ncols;
+                    for (int64_t m = 0; m < ncols; ++m) acc += xr[m] * fp[m];
+                    bcoeffs[(int64_t)j * numindivs + q] = fxscal[j] * acc;
+                }
+                continue;
+            }
+            double co[64 * 64], rr[64], e[64];
+            for (int i = 0; i < K * K; ++i) co[i] = 0.0;
+            for (int i = 0; i < K; ++i) rr[i] = 0.0;
+            int64_t kk = 0;
+            for (int64_t m = 0; m < ncols; ++m) {
+                if (raw[m] < 0.0) continue;
+                for (int j = 0; j < K; ++j) e[j] = fxscal[j] * ffvecs[(int64_t)j * ncols + m];
+                for (int j = 0; j < K; ++j) {
+                    rr[j] += e[j] * xr[m];
+                    for (int k = j; k < K; ++k) co[j * K + k] = co[k * K + j] += e[j] * e[k];
+                }
+                ++kk;
+            }
+            if (kk <= K) { status[q] = 1; continue; }
+            double ans[64];
+            if (chol_solve(co, rr, K, ans) >= 0) memcpy(regans, ans, sizeof(double) * (size_t)K);
+            else status[q] = 2;
+            for (int j = 0; j < K; ++j) acoeffs[(int64_t)j * numindivs + q] = regans[j];
+        }
+    free(raw); free(xr); free(xnum); free(xden);
+}
+
+/* ---- solvit for the ldregress oracle (normal equations of up to 64 unknowns) ------ */
+int orc_solvit(const double *prod, const double *rhs, int32_t n, double *ans)
+{
+    return chol_solve(prod, rhs, n, ans);
+}

@@ -70,6 +70,11 @@ def lib():
         L.orc_lsqproj.argtypes = [u8p, C.c_int64, C.c_int64, C.c_int32, i32p, i32p, f64p,
                                   f64p, f64p, f64p, C.c_int32, f64p, f64p, i32p]
         L.orc_eigscale.argtypes = [f64p, f64p, C.c_int32, i32p, C.c_int32, C.c_int32, f64p]
+        L.orc_lsqproj_fill.argtypes = [u8p, C.c_int64, C.c_int64, C.c_int32, i32p, i32p,
+                                       C.c_int32, i32p, f64p, f64p, f64p, f64p, C.c_int32, f64p,
+                                       f64p, i32p]
+        L.orc_solvit.restype = C.c_int
+        L.orc_solvit.argtypes = [f64p, f64p, C.c_int32, f64p]
         _LIB = L
     return _LIB
 
@@ -215,6 +220,161 @@ def fpca(X, K, L, I, omega):
     B = X.T @ U
     UB, S = np.linalg.svd(B, full_matrices=False)[:2]
     return S[:K] ** 2 / m, sign_rule(UB[:, :K].T)
+
+
+# ---------------------------------------------------------------- non-lookup columns
+def _seqsum(x):
+    """Sum in index order (the reference's plain loop), not numpy's pairwise sum."""
+    x = np.asarray(x, dtype=np.float64)
+    return float(np.add.accumulate(x)[-1]) if x.size else 0.0
+
+
+def normal_columns(rows, xindex, xtypes=None, weight=None, altnorm=1, fancynorm=1,
+                   minallelecnt=1, maxmissing=9999999, live=None):
+    """getcolxz + fvadjust (smartpca.c:3129-3216, 2236-2279), the weight scaling
+    (1150-1152) and the keep rule (1131-1144) for every SNP of rows over the samples
+    xindex.  xtypes (len(xindex)): population index per sample for usepopsformissing.
+    Returns dict(cols [M, n] -- zero for dropped SNPs, status, c0, c1, nmiss (after the
+    fill), mean (filled column), fancy, levels (plain getcolxf levels of kept SNPs))."""
+    g_all = decode_rows(rows, int(np.max(xindex)) + 1)[:, xindex]
+    M, n = g_all.shape
+    out = dict(cols=np.zeros((M, n)), status=np.zeros(M, np.int32), c0=np.zeros(M, np.int32),
+               c1=np.zeros(M, np.int32), nmiss=np.zeros(M, np.int32), mean=np.zeros(M),
+               fancy=np.zeros(M), levels=np.zeros((M, 4)))
+    for m in range(M):
+        if live is not None and not live[m]:
+            out["status"][m] = 3
+            continue
+        g = g_all[m]
+        obs = g >= 0
+        c0 = int(g[obs].sum()); c1 = int((2 - g[obs]).sum())
+        x = g.astype(np.float64)
+        tt = int((~obs).sum())
+        if xtypes is not None and tt > 0:
+            k = np.asarray(xtypes)
+            npop = int(k.max()) + 1
+            psum = np.bincount(k[obs], weights=g[obs], minlength=npop)
+            pnum = np.bincount(k[obs], minlength=npop).astype(np.float64)
+            tt = 0
+            for j in np.nonzero(~obs)[0]:
+                if pnum[k[j]] > 0.5:
+                    x[j] = psum[k[j]] / pnum[k[j]]
+                else:
+                    tt += 1
+        have = x >= 0
+        ynum = float(have.sum())
+        out["c0"][m], out["c1"][m], out["nmiss"][m] = c0, c1, tt
+        if ynum == 0:
+            out["status"][m] = 2
+            continue
+        ysum = _seqsum(x[have])
+        mean = ysum / ynum
+        fancy = 1.0
+        if fancynorm:
+            p = 0.5 * mean if altnorm else (ysum + 1.0) / (2.0 * ynum + 2.0)
+            y = p * (1.0 - p)
+            if y > 0.0:
+                fancy = 1.0 / math.sqrt(y)
+        out["mean"][m], out["fancy"][m] = mean, fancy
+        t = min(c0, c1)
+        if t < minallelecnt or tt > maxmissing or t == 0:
+            out["status"][m] = 1
+            continue
+        col = np.where(have, x - mean, 0.0) * fancy
+        if weight is not None:
+            col = col * weight[m]
+        out["cols"][m] = col
+        # getcolxf's plain levels (no fill)
+        pm = c0 / float(n - int((~obs).sum()))
+        pf = 1.0
+        if fancynorm:
+            p = 0.5 * pm if altnorm else (c0 + 1.0) / (2.0 * (n - int((~obs).sum())) + 2.0)
+            y = p * (1.0 - p)
+            if y > 0.0:
+                pf = 1.0 / math.sqrt(y)
+        out["levels"][m, :3] = (np.arange(3.0) - pm) * pf
+    return out
+
+
+def regressit(eq, rhs):
+    """regressit (regsubs.c:8-74) through solvit/choldc/cholsl (nicksrc/linsubs.c:296-393):
+    eq m x n, normal equations summed in row order; None when not positive definite."""
+    m, n = eq.shape
+    co = np.zeros((n, n)); rr = np.zeros(n)
+    for i in range(m):
+        rr += eq[i] * rhs[i]
+        co += np.outer(eq[i], eq[i])
+    ans = np.zeros(n)
+    ret = lib().orc_solvit(np.ascontiguousarray(co).reshape(-1), np.ascontiguousarray(rr), n, ans)
+    return ans if ret >= 0 else None
+
+
+def ldregress_columns(cols, kept, chrom, ppos, gpos, maxld, ldposlimit=1000000000,
+                      ldlimit=9999.0, r2lo=0.01, r2hi=0.95):
+    """The ldregress loop of smartpca.c:1146-1170 over the kept SNPs in order: ldregx
+    (2751-2826) on the reference's buffer, then bumpldvv (2691-2748).  cols [M, n]
+    normalised columns, kept: SNP indices in order.  Returns (residual columns [len(kept), n],
+    ynumsnps)."""
+    n = cols.shape[1]
+    buf, ids = [], []
+    lastc = None
+    res = np.zeros((len(kept), n))
+    ynum = 0.0
+    for k, m in enumerate(kept):
+        cc = cols[m]
+        if lastc != chrom[m]:
+            buf, ids = [], []
+        # ldregx
+        rc, passed = 0, []
+        for src in buf:
+            ssum = 0.0
+            for ii in range(n):  # same order as the reference's loop
+                ssum += cc[ii] * src[ii]
+            r2 = (ssum / (2 * (n - 1))) ** 2
+            if r2 > r2hi:
+                rc = 2
+                break
+            if r2 > r2lo:
+                passed.append(src)
+        if rc == 2:
+            ww = np.zeros(n)
+        elif passed:
+            S = np.array(passed)  # rsize_pass x n
+            ans = regressit(S.T, cc)
+            regans = ans if ans is not None else np.zeros(len(passed))
+            www = np.zeros(n)
+            for a in range(len(passed)):
+                www += regans[a] * S[a]
+            ww = cc - www
+            rc = 1
+        else:
+            ww = cc.copy()
+        if rc < 2:
+            ynum += float(np.dot(ww, ww)) / float(np.dot(cc, cc))
+        newsrc = cc if rc < 2 else ww
+        # bumpldvv: drop the oldest while out of range, append (dropping one when full)
+        while buf:
+            f = ids[0]
+            pdiff = int(np.rint(ppos[m] - ppos[f]))
+            gdiff = gpos[m] - gpos[f]
+            if pdiff <= ldposlimit and gdiff <= ldlimit:
+                break
+            buf.pop(0); ids.pop(0)
+        if len(buf) == maxld:
+            buf.pop(0); ids.pop(0)
+        buf.append(newsrc.copy()); ids.append(m)
+        lastc = chrom[m]
+        res[k] = ww
+    return res, ynum
+
+
+def dense_xtx(cols):
+    """domult_increment_normal (smartpca.c:3498-3561) + symit2 + trace scaling
+    (480-508, 1222-1236) of the columns [ncols, n]: returns (full XTX / y, y)."""
+    X = cols.T @ cols
+    n = X.shape[0]
+    y = float(np.trace(X)) / (n - 1)
+    return X / y, y
 
 
 def finalize_xtx(lower, n):
@@ -398,6 +558,15 @@ def smartpca(par: dict, base_dir: str = ".", geno_rows=None, nthreads=8, verbose
     numchrom = int(P("numchrom", 22))
     hiprec = _yes(P("hiprecision", P("hiprec", "NO")))
     regmode = _yes(P("lsqproject", P("regmode", "YES")))
+    usepops = _yes(P("usepopsformissing", "NO"))
+    ldregress = int(P("ldregress", P("nsnpldregress", 0)))
+    ldlimit = float(P("maxdistldregress", P("ldlimit", 9999.0)))
+    ldposlimit = int(P("ldposlimit", 1000000000))
+    ldr2lo, ldr2hi = float(P("ldr2lo", 0.01)), float(P("ldr2hi", 0.95))
+    if ldlimit <= 0 or ldposlimit <= 0:
+        ldregress = 0
+    weightname = P("weightname")
+    normal = usepops or ldregress > 0 or weightname is not None
 
     snp = read_snp(path("snpname"))
     ind = read_ind(path("indivname"))
@@ -438,6 +607,29 @@ def smartpca(par: dict, base_dir: str = ".", geno_rows=None, nthreads=8, verbose
     ind_ignore |= vv == 0
     keep_snps = np.nonzero(~snp_ignore)[0]
     rows = geno_rows[keep_snps]
+    # axis populations in first-appearance order (makeeglist) and each sample's index
+    if poplist:
+        eglist = [l.split()[0] for l in open(path("poplistname")) if l.split() and not l.startswith("#")]
+    else:
+        eglist = []
+        for i in range(N):
+            if not ind_ignore[i] and ind["pop"][i] not in eglist:
+                eglist.append(ind["pop"][i])
+    popidx = np.array([eglist.index(p) if p in eglist else -1 for p in ind["pop"]], np.int32)
+    xt_quirk = np.full(N, -2, np.int32)
+    weight = None
+    if weightname:
+        ids_k = [snp["ids"][m] for m in keep_snps]
+        weight = np.ones(len(keep_snps))
+        wp = weightname if os.path.isabs(weightname) else os.path.join(base_dir, weightname)
+        for line in open(wp):
+            f = line.split()
+            if not f or f[0].startswith("#") or f[0] in ("SNP_ID", "Indiv_ID", "Chr"):
+                continue
+            if len(f) > 1 and f[0] in ids_k:
+                weight[ids_k.index(f[0])] = float(f[1])
+    ynumsnps = 0.0
+
     g_k = g_all[keep_snps]
     Mk = len(keep_snps)
     snp_dead = np.zeros(Mk, dtype=bool)  # cupt->ignore set during passes
@@ -451,11 +643,29 @@ def smartpca(par: dict, base_dir: str = ".", geno_rows=None, nthreads=8, verbose
         # loadsnpx: live SNPs with >= 1 valid genotype over non-ignored indivs
         live = ~snp_dead & (((g_k >= 0) & (~ind_ignore)[None, :]).sum(axis=1) > 0)
         cols = np.nonzero(live)[0]
-        st = snpstats(rows[cols], xindex, altnorm, fancynorm, minallelecnt, maxmissing)
-        killed = st["status"] != 0
-        snp_dead[cols[killed]] = True
-        lower = xtx_lookup(rows[cols], xindex, st["levels"], st["status"], nthreads)
-        full, y = finalize_xtx(lower, nrows)
+        xt_quirk[:nrows] = popidx[xindex]
+        if normal:
+            st = normal_columns(rows[cols], xindex, popidx[xindex] if usepops else None,
+                                None if weight is None else weight[cols], altnorm, fancynorm,
+                                minallelecnt, maxmissing)
+            killed = st["status"] != 0
+            snp_dead[cols[killed]] = True
+            kc = np.nonzero(~killed)[0]
+            if ldregress > 0:
+                ks = keep_snps[cols]
+                Z, ynumsnps = ldregress_columns(
+                    st["cols"], kc, snp["chrom"][ks], snp["ppos"][ks], snp["gpos"][ks],
+                    ldregress, ldposlimit, ldlimit, ldr2lo, ldr2hi)
+            else:
+                Z, ynumsnps = st["cols"][kc], float(len(kc))
+            full, y = dense_xtx(Z)
+        else:
+            st = snpstats(rows[cols], xindex, altnorm, fancynorm, minallelecnt, maxmissing)
+            killed = st["status"] != 0
+            snp_dead[cols[killed]] = True
+            lower = xtx_lookup(rows[cols], xindex, st["levels"], st["status"], nthreads)
+            full, y = finalize_xtx(lower, nrows)
+            ynumsnps = float(int((~killed).sum()))
         if y == -1.0:
             raise RuntimeError("bad XTX matrix")
         if y == -2.0:
@@ -476,7 +686,7 @@ def smartpca(par: dict, base_dir: str = ".", geno_rows=None, nthreads=8, verbose
     K = min(numeigs, nrows, ncols)
     ev_sign = sign_rule(evecs[:K])
     res = dict(evals=evals, evecs=ev_sign, xindex=xindex, removed=removed, K=K,
-               xtx=full, trace_scale=y, stats=st, cols=keep_snps[cols])
+               xtx=full, trace_scale=y, stats=st, cols=keep_snps[cols], ynumsnps=ynumsnps)
     eval_lines = ["%12.6f" % v for v in evals]
     head = "%20s " % "#eigvals:" + "".join("%9.3f " % evals[j] for j in range(K))
     if K == 0:
@@ -496,9 +706,14 @@ def smartpca(par: dict, base_dir: str = ".", geno_rows=None, nthreads=8, verbose
     acoeffs = np.zeros((K, N)); bcoeffs = np.zeros((K, N)); status = np.zeros(N, np.int32)
     use = (~ind_ignore).astype(np.int32)
     if regmode:
-        L.orc_lsqproj(crow.reshape(-1), rlen, ncols, N, use, ignore_c, xmean, xfancy,
-                      ffvecs.reshape(-1), fxscal, K, acoeffs.reshape(-1), bcoeffs.reshape(-1),
-                      status)
+        if usepops:
+            L.orc_lsqproj_fill(crow.reshape(-1), rlen, ncols, N, use, np.ascontiguousarray(xt_quirk),
+                               len(eglist), ignore_c, xmean, xfancy, ffvecs.reshape(-1), fxscal, K,
+                               acoeffs.reshape(-1), bcoeffs.reshape(-1), status)
+        else:
+            L.orc_lsqproj(crow.reshape(-1), rlen, ncols, N, use, ignore_c, xmean, xfancy,
+                          ffvecs.reshape(-1), fxscal, K, acoeffs.reshape(-1),
+                          bcoeffs.reshape(-1), status)
         eigscale = np.zeros(K)
         L.orc_eigscale(acoeffs.reshape(-1), bcoeffs.reshape(-1), N, xindex, nrows, K, eigscale)
         acoeffs = acoeffs * eigscale[:, None]
