@@ -45,6 +45,7 @@ SIGNATURES = {
                                  C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32, _vp,
                                  _vp, _vp, _vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                  _vp]),
+    "eig_popcounts": (C.c_int, [_vp, _i32p, C.c_int32, _vp, _vp, _i32p]),
     "eig_eigh_xtx": (C.c_int, [_vp, C.c_int32, _f64p, _vp]),
     "eig_eigh": (C.c_int, [_vp, _f64p, C.c_int32, C.c_int32, _f64p, _vp]),
     "eig_loadings": (C.c_int, [_vp, _f64p, C.c_int32, _vp, _vp, _vp]),
@@ -222,6 +223,15 @@ class Engine:
                                          _ptr(st["nmiss"]), _ptr(st["status"]), C.byref(y),
                                          C.byref(yn), _ptr(mat)), "eig_xtx_normal")
         return y.value, yn.value, st, mat
+
+    def popcounts(self, xtypes, npop, haploid=None, xsnp=None):
+        xt = np.ascontiguousarray(xtypes, dtype=np.int32)
+        h = None if haploid is None else np.ascontiguousarray(haploid, dtype=np.uint8)
+        x = None if xsnp is None else np.ascontiguousarray(xsnp, dtype=np.uint8)
+        out = np.zeros((self.nsnp, npop, 2), np.int32)
+        self._chk(self._L.eig_popcounts(self.h, xt, npop, _ptr(h), _ptr(x), out.reshape(-1)),
+                  "eig_popcounts")
+        return out
 
     def eigh_xtx(self, nvec):
         ev = np.zeros(self.nrows)

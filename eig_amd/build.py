@@ -76,17 +76,26 @@ def build(verbose: bool = False) -> str:
 
 
 def build_cli() -> None:
-    """The smartpca drop-in: C host code linked against libeig_amd.so."""
-    srcs = sorted(os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".c")) \
-        if os.path.isdir(HOST) else []
-    if not srcs:
+    """The smartpca drop-in (C host code linked against libeig_amd.so), the twstats
+    tool, and the host statistics as a library of their own for the CPU tests."""
+    if not os.path.isdir(HOST):
         return
+    srcs = sorted(os.path.join(HOST, f) for f in os.listdir(HOST)
+                  if f.endswith(".c") and f != "twstats_main.c")
     os.makedirs(BIN, exist_ok=True)
+    cflags = ["gcc", "-O2", "-std=c11", "-Wall", "-Wno-unused-result", "-D_GNU_SOURCE",
+              "-I" + os.path.join(ROOT, "include")]
     exe = os.path.join(BIN, "smartpca")
-    _run(["gcc", "-O2", "-std=c11", "-Wall", "-Wno-unused-result", "-D_GNU_SOURCE",
-          "-I" + os.path.join(ROOT, "include"), "-o", exe + ".tmp"] + srcs +
+    _run(cflags + ["-o", exe + ".tmp"] + srcs +
          ["-L" + PKG, "-leig_amd", "-Wl,-rpath,$ORIGIN/..", "-lm", "-lpthread"])
     os.replace(exe + ".tmp", exe)
+    tw = os.path.join(BIN, "twstats")
+    _run(cflags + ["-o", tw + ".tmp", os.path.join(HOST, "twstats_main.c"),
+                   os.path.join(HOST, "stats.c"), os.path.join(HOST, "parfile.c"), "-lm"])
+    os.replace(tw + ".tmp", tw)
+    so = os.path.join(PKG, "libeig_hoststats.so")
+    _run(cflags + ["-fPIC", "-shared", "-o", so + ".tmp", os.path.join(HOST, "stats.c"), "-lm"])
+    os.replace(so + ".tmp", so)
 
 
 if __name__ == "__main__":

@@ -374,6 +374,44 @@ __global__ __launch_bounds__(256) void k_lsq_filltab(const uint8_t *__restrict__
     fill[m * npop + k] = pnum[k] > 0 ? (double)psum[k] / (double)pnum[k] : __builtin_nan("");
 }
 
+// Allele counts per SNP and population over the selected samples (one block per SNP):
+// out[(m * npop + t) * 2 + {0, 1}] = sum of g and of 2 - g over the observed samples of
+// population t; on an X-chromosome SNP a haploid sample counts g/2 and 1 - g/2 and a
+// heterozygous call of one is skipped (getrawcolx, qpsubs.c:251-283; fstcolyy 1205-1262)
+__global__ __launch_bounds__(256) void k_popcounts(const uint32_t *__restrict__ sel, int64_t M,
+                                                   int32_t nrows,
+                                                   const int32_t *__restrict__ xtypes,
+                                                   int32_t npop,
+                                                   const uint8_t *__restrict__ haploid,
+                                                   const uint8_t *__restrict__ xsnp,
+                                                   int32_t *__restrict__ out) {
+  __shared__ int32_t ca[kMaxPop], cb[kMaxPop];
+  const int64_t m = blockIdx.x;
+  const int t = threadIdx.x;
+  for (int k = t; k < npop; k += 256) ca[k] = cb[k] = 0;
+  __syncthreads();
+  const bool onx = xsnp && xsnp[m];
+  for (int32_t i = t; i < nrows; i += 256) {
+    const int32_t ty = xtypes[i];
+    if (ty < 0 || ty >= npop) continue;
+    int32_t g = (int32_t)sel_code(sel, M, i, m);
+    if (g == 3) continue;
+    int32_t a = g, b = 2 - g;
+    if (onx && haploid && haploid[i]) {
+      if (g == 1) continue;
+      a = g / 2;
+      b = 1 - a;
+    }
+    atomicAdd(&ca[ty], a);
+    atomicAdd(&cb[ty], b);
+  }
+  __syncthreads();
+  for (int k = t; k < npop; k += 256) {
+    out[(m * npop + k) * 2] = ca[k];
+    out[(m * npop + k) * 2 + 1] = cb[k];
+  }
+}
+
 // observed genotypes of each individual over the SNPs not ignored (numvalids,
 // admutils.c:312-340), one thread per individual
 __global__ __launch_bounds__(256) void k_lsq_count(const uint8_t *__restrict__ geno, int64_t rlen,
@@ -616,5 +654,34 @@ extern "C" int eig_xtx_normal(eig_ctx *ctx, const int32_t *xtypes, int32_t npop,
   }
   EIG_HIP(ctx, hipStreamSynchronize(st));
   ctx->have_xtx = true;
+  return EIG_OK;
+}
+
+extern "C" int eig_popcounts(eig_ctx *ctx, const int32_t *xtypes, int32_t npop,
+                             const uint8_t *haploid, const uint8_t *xsnp, int32_t *counts) {
+  if (!ctx || !xtypes || !counts) return eig_set_err(ctx, EIG_E_ARG, "eig_popcounts: args");
+  if (!ctx->have_sel) return eig_set_err(ctx, EIG_E_STATE, "eig_popcounts before eig_select");
+  if (npop <= 0 || npop > kMaxPop)
+    return eig_set_err(ctx, EIG_E_ARG, "eig_popcounts: 1 <= npop <= %d", kMaxPop);
+  EIG_HIP(ctx, hipSetDevice(ctx->device));
+  const int64_t M = ctx->nsnp;
+  const int32_t n = ctx->nrows;
+  hipStream_t st = ctx->stream;
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t o_xt = 0, o_h = up(4 * (size_t)n), o_x = o_h + up((size_t)n),
+               o_out = o_x + up((size_t)M);
+  const size_t out_b = 8 * (size_t)M * npop;
+  EIG_TRY(eig_ensure(ctx, ctx->scratch, o_out + out_b));
+  char *base = (char *)ctx->scratch.p;
+  EIG_HIP(ctx, hipMemcpyAsync(base + o_xt, xtypes, 4 * (size_t)n, hipMemcpyHostToDevice, st));
+  if (haploid) EIG_HIP(ctx, hipMemcpyAsync(base + o_h, haploid, (size_t)n, hipMemcpyHostToDevice, st));
+  if (xsnp) EIG_HIP(ctx, hipMemcpyAsync(base + o_x, xsnp, (size_t)M, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_popcounts, dim3((unsigned)M), dim3(256), 0, st, (const uint32_t *)ctx->sel.p,
+                     M, n, (const int32_t *)(base + o_xt), npop,
+                     haploid ? (const uint8_t *)(base + o_h) : nullptr,
+                     xsnp ? (const uint8_t *)(base + o_x) : nullptr, (int32_t *)(base + o_out));
+  EIG_HIP(ctx, hipGetLastError());
+  EIG_HIP(ctx, hipMemcpyAsync(counts, base + o_out, out_b, hipMemcpyDeviceToHost, st));
+  EIG_HIP(ctx, hipStreamSynchronize(st));
   return EIG_OK;
 }

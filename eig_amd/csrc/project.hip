@@ -10,6 +10,7 @@
 //             e_m = fxscal .* ffvecs[:,m] and the right-hand side sum_m e_m x_m,
 //             reduced over fixed SNP chunks in chunk order, then the Cholesky
 //             solve of nicksrc/linsubs.c (choldc/cholsl) per individual.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -182,11 +183,12 @@ __global__ __launch_bounds__(64) void k_lsq_gram(const uint8_t *__restrict__ gen
                                                  const uint8_t *__restrict__ ignore,
                                                  const double *__restrict__ fill,
                                                  const int32_t *__restrict__ itype, int32_t npop,
-                                                 double *__restrict__ partial, int32_t nblk) {
+                                                 double *__restrict__ partial, int32_t nblk,
+                                                 int64_t chunk0) {
   __shared__ double se[64][kMaxK + 2];
   const int li = threadIdx.x;
   const int32_t q = (blockIdx.x % nblk) * 64 + li;
-  const int64_t chunk = blockIdx.x / nblk;
+  const int64_t chunk = chunk0 + blockIdx.x / nblk;
   const int64_t m0 = chunk * kLsqChunk, m1 = min(M, m0 + kLsqChunk);
   constexpr int nP = K * (K + 1) / 2;
   double G[nP];
@@ -241,7 +243,7 @@ __global__ __launch_bounds__(64) void k_lsq_gram(const uint8_t *__restrict__ gen
     }
   }
   if (q < nind) {
-    double *out = partial + (chunk * (int64_t)nind + q) * (nP + K + 1);
+    double *out = partial + ((chunk - chunk0) * (int64_t)nind + q) * (nP + K + 2);
 #pragma unroll
     for (int p = 0; p < nP; ++p) out[p] = G[p];
 #pragma unroll
@@ -410,7 +412,7 @@ static int lsqproject(eig_ctx *ctx, const int32_t *use, const int32_t *itype, in
                                     (const double *)ctx->ffvecs.p, d_fx,
                                     (const double *)ctx->fancy.p, d_xm,
                                     (const uint8_t *)ctx->ignore_dev.p, (const double *)d_fill,
-                                    d_itype, npop, (double *)ctx->scratch2.p, nblk));
+                                    d_itype, npop, (double *)ctx->scratch2.p, nblk, (int64_t)0));
   hipLaunchKernelGGL(k_lsq_solve, dim3((unsigned)((nind + 127) / 128)), dim3(128), 0, st,
                      (const double *)ctx->scratch2.p, nchunk, nind, K, (const int32_t *)d_use,
                      d_itype, d_ans, d_rhs, d_st);

@@ -21,6 +21,7 @@
 #include <unistd.h>
 
 #include "smartpca.h"
+#include "stats.h"
 
 void die(const char *fmt, ...) {
   va_list ap;
@@ -42,6 +43,9 @@ void die(const char *fmt, ...) {
 typedef struct {
   char *genotypename, *snpname, *indivname, *poplistname, *outputname, *outputvname;
   char *outliername, *snpeigname, *grmoutname, *deletesnpoutname, *xtxform, *weightname;
+  char *phylipname, *fstdetailsname, *megaoutname;
+  int nostatslim, fsthiprec, fstverbose, fstz, pubmean;
+  double blgsize;
   int numeigs, numoutliter, numoutleigs, outliermode, altnormstyle, fancynorm;
   int minallelecnt, maxmissing, noxdata, numchrom, hiprec, regmode, hashcheck;
   int familynames, doproject, malexhet, grmbinary, xchrom, zchrom, lopos, hipos;
@@ -79,6 +83,9 @@ static void defaults(Settings *s) {
   s->ldr2lo = 0.01;
   s->ldr2hi = 0.95;
   s->ldposlimit = 1000 * 1000 * 1000;
+  s->nostatslim = 10;           /* smartpca.c:183 */
+  s->blgsize = 0.05;            /* smartpca.c:251, Morgans */
+  s->pubmean = 1;               /* smartpca.c:275 */
 }
 
 static void read_settings(const Params *pp, Settings *s) {
@@ -147,6 +154,17 @@ static void read_settings(const Params *pp, Settings *s) {
   par_dbl(pp, "ldr2lo:", &s->ldr2lo);
   par_dbl(pp, "ldr2hi:", &s->ldr2hi);
   par_dbl(pp, "maxdistldregress:", &s->ldlimit);
+  par_string(pp, "phylipname:", &s->phylipname);
+  par_string(pp, "phylipoutname:", &s->phylipname);
+  par_string(pp, "fstdetailsname:", &s->fstdetailsname);
+  par_string(pp, "megaoutname:", &s->megaoutname);
+  par_int(pp, "nostatslim:", &s->nostatslim);
+  par_int(pp, "minleneig:", &s->nostatslim);
+  par_int(pp, "fsthiprecision:", &s->fsthiprec);
+  par_int(pp, "fstverbose:", &s->fstverbose);
+  par_int(pp, "fstz:", &s->fstz);
+  par_int(pp, "pubmean:", &s->pubmean);
+  par_dbl(pp, "blgsize:", &s->blgsize);
   par_string(pp, "xtxform:", &s->xtxform);
   par_int(pp, "bf16block:", &s->bf16block);
   par_int(pp, "gpu:", &s->gpu);
@@ -163,7 +181,8 @@ static void reject_unsupported(const Params *pp) {
       "genooutfilename:", "genotypeoutname:", NULL};
   static const char *const nonzero[] = {
       "killr2:", "shrinkmode:", "newshrink:", "fastshrink:", "autoshrink:", "mpshrink:",
-      "missingmode:", "easymode:", "fstonly:", "qtmode:", "rounakmode:", NULL};
+      "missingmode:", "easymode:", "fstonly:", "qtmode:", "rounakmode:", "inbreed:",
+      "markerscore:", "pcorrmode:", "dotpopsmode:", "chisqmode:", NULL};
   for (int i = 0; any_value[i]; ++i)
     if (par_str(pp, any_value[i]))
       die("parameter %s is not supported by this build", any_value[i]);
@@ -669,15 +688,17 @@ int main(int argc, char **argv) {
     printf("grm dumped\n");
   }
 
-  /* ---- outputs (smartpca.c:1370-1591) */
+  /* ---- outputs (smartpca.c:1300-1591) */
   int K = S.numeigs;
   if (K > nrows) K = nrows;
   if (K > ncols) K = (int)ncols;
   int nvec_got = nvec_pass < nrows ? nvec_pass : nrows;
   if (K > nvec_got) K = nvec_got;
-  int npos = 0;
-  for (int32_t i = 0; i < nrows; ++i) npos += evals[i] > 0.0;
-  if (npos == 0) die("no data");
+  const int mpos = st_numgtz(evals, nrows);
+  if (mpos == 0) die("no data");
+  /* Tracy-Widom statistics of the positive eigenvalues (smartpca.c:1312-1368) */
+  printf("\n## Tracy-Widom statistics: rows: %d  cols: %lld\n", nrows, (long long)ncols);
+  st_print_tw(stdout, evals, mpos, -1.0, (double)nrows, S.nostatslim < 3 ? 3 : S.nostatslim);
   if (S.outputvname) {
     FILE *f = fopen(S.outputvname, "w");
     if (!f) die("can't open %s", S.outputvname);
@@ -689,15 +710,32 @@ int main(int argc, char **argv) {
     ofile = fopen(S.outputname, "w");
     if (!ofile) die("can't open %s", S.outputname);
   }
+  uint8_t *ign = malloc((size_t)Mk);
+  for (int64_t m = 0; m < Mk; ++m) ign[m] = dead[m] || status[m] != EIG_SNP_KEPT;
+  double *ffvecs = NULL;
   if (K > 0) {
     fprintf(ofile, "%20s ", "#eigvals:");
     for (int j = 0; j < K; ++j) fprintf(ofile, "%9.3f ", evals[j]);
     fprintf(ofile, "\n");
     /* loadings + projection scales over the SNPs that survived the final pass */
-    uint8_t *ign = malloc((size_t)Mk);
-    for (int64_t m = 0; m < Mk; ++m) ign[m] = dead[m] || status[m] != EIG_SNP_KEPT;
-    double *ffvecs = malloc(sizeof(double) * (size_t)K * Mk), fxscal[64];
+    ffvecs = malloc(sizeof(double) * (size_t)K * Mk);
+    double fxscal[64];
     CHK(eig_loadings(ctx, evecs, K, ign, ffvecs, fxscal));
+    /* kurtosis of each axis's SNP loadings (over the pass's SNPs) and sample
+       coordinates (smartpca.c:1497-1501, 1531-1538) */
+    {
+      double *w = malloc(sizeof(double) * (size_t)(ncols > nrows ? ncols : nrows));
+      printf("%12s %4s %9s %9s\n", "kurtosis", "", "snps", "indivs");
+      for (int j = 0; j < K; ++j) {
+        int64_t c = 0;
+        for (int64_t m = 0; m < Mk; ++m)
+          if (live[m]) w[c++] = ffvecs[(int64_t)j * Mk + m];
+        const double ks = st_kurtosis(w, (int)c);
+        const double ki = st_kurtosis(evecs + (int64_t)j * nrows, nrows);
+        printf("%12s %4d %9.3f %9.3f\n", "eigenvector", j + 1, ks, ki);
+      }
+      free(w);
+    }
     double *acoeffs = calloc((size_t)K * N, sizeof(double));
     double *bcoeffs = calloc((size_t)K * N, sizeof(double));
     if (S.regmode) {
@@ -735,6 +773,143 @@ int main(int argc, char **argv) {
         fprintf(ofile, S.hiprec ? "%12.6f  " : "%10.4f  ", acoeffs[(int64_t)j * N + i]);
       fprintf(ofile, "%15s\n", inds[i].pop);
     }
+    free(acoeffs);
+    free(bcoeffs);
+  }
+  if (ofile != stdout) fclose(ofile);
+
+  /* ---- population statistics (smartpca.c:1779-1994) */
+  int *xpopsize = calloc(neg > 0 ? neg : 1, sizeof(int));
+  for (int32_t r = 0; r < nrows; ++r) ++xpopsize[xtypes[r]];
+  for (int k = 0; k < neg; ++k) {
+    printf("population: %3d %20s %4d", k, eglist[k], xpopsize[k]);
+    if (xpopsize[k] == 0) printf(" ***");
+    printf("\n");
+  }
+  {
+    /* Fst between the axis populations with a weighted block jackknife over blocks of
+       blgsize Morgans (dofstxx, smartpca.c:2830-2861; dofstnumx, qpsubs.c:2488-2740) */
+    const int G2 = neg * neg;
+    int32_t *cnt = malloc(sizeof(int32_t) * (size_t)Mk * neg * 2);
+    uint8_t *hap = malloc((size_t)nrows), *xs = malloc((size_t)Mk);
+    for (int32_t r = 0; r < nrows; ++r) hap[r] = inds[xindex[r]].sex == 'M';
+    for (int64_t m = 0; m < Mk; ++m) xs[m] = snps[m].chrom == 23;
+    CHK(eig_popcounts(ctx, xtypes, neg, hap, xs, cnt));
+    int *tag = malloc(sizeof(int) * (size_t)Mk), *cm = malloc(sizeof(int) * (size_t)Mk);
+    double *gp = malloc(sizeof(double) * (size_t)Mk);
+    for (int64_t m = 0; m < Mk; ++m) { cm[m] = snps[m].chrom; gp[m] = snps[m].gpos; }
+    const int nblocks = st_setblocks(cm, gp, ign, Mk, S.blgsize, tag);
+    printf("number of blocks for block jackknife: %d\n", nblocks);
+    double *fmean = malloc(sizeof(double) * G2), *fans = malloc(sizeof(double) * G2),
+           *fsd = malloc(sizeof(double) * G2), *fzz = malloc(sizeof(double) * G2);
+    int *fnum = malloc(sizeof(int) * G2);
+    st_fst(cnt, ign, tag, Mk, neg, nblocks, fmean, fans, fsd, fnum);
+    for (int k = 0; k < G2; ++k) fzz[k] = fans[k];
+    for (int k1 = 0; k1 < neg; ++k1)
+      for (int k2 = k1 + 1; k2 < neg; ++k2) {
+        double z = fans[k1 * neg + k2] / (fsd[k1 * neg + k2] + 1.0e-10);
+        z = z < -9.999999 ? -9.999999 : (z > 9.999999 ? 9.999999 : z);
+        fzz[k2 * neg + k1] = z;
+      }
+    if (S.megaoutname) { /* printmega (smartpca.c:4785-4812) */
+      FILE *f = fopen(S.megaoutname, "w");
+      if (!f) die("can't open %s", S.megaoutname);
+      fprintf(f, "#mega\n!TITLE distance for %d samples ;\n\n", neg);
+      for (int k = 0; k < neg; ++k) fprintf(f, "#%s\n", eglist[k]);
+      fprintf(f, "\n\n");
+      for (int a = 0; a < neg; ++a) {
+        for (int b = 0; b < a; ++b) fprintf(f, "  %9.6f", fans[a * neg + b] > 0 ? fans[a * neg + b] : 0.0);
+        fprintf(f, "\n");
+      }
+      fclose(f);
+    }
+    if ((!S.phylipname && neg > 10) || S.fstverbose) {
+      printf("## Fst statistics between populations:         fst       std error      Z\n");
+      for (int k1 = 0; k1 < neg; ++k1)
+        for (int k2 = k1 + 1; k2 < neg; ++k2) {
+          const double z = fans[k1 * neg + k2] / (fsd[k1 * neg + k2] + 1.0e-10);
+          if (!S.fsthiprec)
+            printf(" %20s %20s %9.3f %10.4f", eglist[k1], eglist[k2], fans[k1 * neg + k2],
+                   fsd[k1 * neg + k2]);
+          else
+            printf(" %20s %20s %12.6f %12.6f", eglist[k1], eglist[k2], fans[k1 * neg + k2],
+                   fsd[k1 * neg + k2]);
+          printf("  %9.3f ", z);
+          printf(" %12d\n", fnum[k1 * neg + k2]);
+        }
+      printf("\n");
+    }
+    printf("## end of Fst statistics between populations\n");
+    if (S.fstdetailsname) {
+      FILE *f = fopen(S.fstdetailsname, "w");
+      if (!f) die("can't open %s", S.fstdetailsname);
+      fprintf(f, "## Fst statistics between populations:     fst     fstjack       std error      Z    snpnumber\n");
+      for (int k1 = 0; k1 < neg; ++k1)
+        for (int k2 = k1 + 1; k2 < neg; ++k2) {
+          const int k = k1 * neg + k2;
+          fprintf(f, S.fsthiprec ? "F_st %20s %20s %15.9f %15.9f %15.9f"
+                                 : "F_st %20s %20s %12.6f %12.6f %12.6f",
+                  eglist[k1], eglist[k2], fmean[k], fans[k], fsd[k]);
+          fprintf(f, " %9.3f", fans[k] / (fsd[k] + 1.0e-10));
+          fprintf(f, " %6d\n", fnum[k]);
+        }
+      fclose(f);
+    }
+    if (S.fstz) {
+      printf("\n");
+      printf(S.fsthiprec ? "Fst: Z/F *1000000 [clipped to -10,10]\n"
+                         : "Fst: Z/F *1000 [clipped to -10,10]\n");
+      st_printmatz(fzz, eglist, neg, S.fsthiprec ? 1e6 : 1e3, S.fsthiprec ? 10 : 5);
+    }
+    printf("\n");
+    if (S.phylipname) {
+      FILE *f = fopen(S.phylipname, "w");
+      if (!f) die("can't open %s", S.phylipname);
+      fprintf(f, "%6d\n", neg);
+      for (int k1 = 0; k1 < neg; ++k1) {
+        char nm[11];
+        snprintf(nm, sizeof nm, "%s", eglist[k1]);
+        fprintf(f, "%10s", nm);
+        for (int k2 = 0; k2 < neg; ++k2)
+          fprintf(f, "%6.3f", 0.5 * (fans[k1 * neg + k2] + fans[k2 * neg + k1]));
+        fprintf(f, "\n");
+      }
+      fclose(f);
+    }
+    if (neg <= 10) {
+      printf(S.fsthiprec ? "fst *1000000:\n" : "fst *1000:\n");
+      st_printmatz(fans, eglist, neg, S.fsthiprec ? 1e6 : 1e3, S.fsthiprec ? 10 : 5);
+      printf("\n");
+    }
+    printf("std. err.  * 1000000:\n");
+    st_printmatz(fsd, eglist, neg, 1e6, 5);
+    printf("\n");
+    free(cnt); free(hap); free(xs); free(tag); free(cm); free(gp);
+    free(fmean); free(fans); free(fsd); free(fzz); free(fnum);
+  }
+  /* anova of each eigenvector by population, and the population differences over the
+     K axes (dottest, smartpca.c:1957-1994, 2315-2413) */
+  double *chitot = calloc((size_t)(neg > 0 ? neg * neg : 1), sizeof(double));
+  for (int j = 0; j < K; ++j) {
+    char sss[64];
+    snprintf(sss, sizeof sss, "eigenvector %d", j + 1);
+    st_dottest(sss, evecs + (int64_t)j * nrows, eglist, neg, xtypes, nrows, chitot, S.pubmean);
+  }
+  printf("\n## Statistical significance of differences beween populations:\n");
+  printf("                                pop1                  pop2      chisq          p-value   |pop1|   |pop2|\n");
+  for (int k1 = 0; k1 < neg; ++k1)
+    for (int k2 = k1 + 1; k2 < neg; ++k2) {
+      const double ychi = chitot[k1 * neg + k2];
+      printf("popdifference:  %20s  %20s  %12.3f  %12.6g", eglist[k1], eglist[k2], ychi,
+             st_chitail(K, ychi));
+      printf("   %5d", xpopsize[k1]);
+      printf("   %5d\n", xpopsize[k2]);
+    }
+  printf("\n");
+  free(chitot);
+  free(xpopsize);
+
+  if (K > 0) {
     /* snp weights (writesnpeigs, smartpca.c:3038-3104): loadings scaled to mean square
        1 over the pass's SNPs, the 10 largest |weights| of each axis on stdout, then the
        weights file */
@@ -774,12 +949,9 @@ int main(int argc, char **argv) {
       }
       fclose(f);
     }
-    free(ign);
-    free(ffvecs);
-    free(acoeffs);
-    free(bcoeffs);
   }
-  if (ofile != stdout) fclose(ofile);
+  free(ign);
+  free(ffvecs);
   printf("##end of smartpca: %12.3f seconds wall (xtx %.3f, eigensolve %.3f)\n",
          now_s() - t_start, t_xtx, t_eig);
   eig_close(ctx);

@@ -134,6 +134,15 @@ int eig_xtx_normal(eig_ctx *ctx, const int32_t *xtypes, int32_t npop, const doub
                    int32_t *nmiss, int32_t *status, double *trace_scale, double *ynumsnps,
                    double *xtx_out);
 
+/* Allele counts per SNP and population over the selected samples, for the Fst
+ * statistics (fstcolyy + getrawcolx, qpsubs.c:1205-1262, 251-283): counts[(m * npop + t)
+ * * 2] = sum of g and counts[... + 1] = sum of 2 - g over the observed samples of
+ * population t = xtypes[i] (nrows entries; < 0 = none).  On a SNP with xsnp[m] != 0
+ * a sample with haploid[i] != 0 counts g/2 and 1 - g/2, and its heterozygous call is
+ * skipped.  haploid and xsnp may be NULL.  counts: nsnp * npop * 2 entries. */
+int eig_popcounts(eig_ctx *ctx, const int32_t *xtypes, int32_t npop, const uint8_t *haploid,
+                  const uint8_t *xsnp, int32_t *counts);
+
 /* Eigen-decomposition of the device-resident normalised XTX: eigvecs
  * (eigsubs.c:39-55) without its LAPACK call.  evals (nrows, descending) get every
  * eigenvalue; evecs (nvec x nrows, row j = eigenvector j, unit norm) the top

@@ -38,11 +38,11 @@ def eng():
 
 
 def _max_sin_angle(a, b):
-    """Largest principal-angle sine between the row spaces of a and b (k x n, orthonormal rows)."""
+    """Largest principal-angle sine between the row spaces of a and b (k x n):
+    || (I - Qa Qa^T) Qb ||_2, accurate for small angles (unlike sqrt(1 - cos^2))."""
     qa, _ = np.linalg.qr(a.T)
     qb, _ = np.linalg.qr(b.T)
-    s = np.linalg.svd(qa.T @ qb, compute_uv=False)
-    return float(np.sqrt(max(0.0, 1.0 - s.min() ** 2)))
+    return float(np.linalg.norm(qb - qa @ (qa.T @ qb), 2))
 
 
 @pytest.fixture(scope="module")

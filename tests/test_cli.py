@@ -252,3 +252,31 @@ def test_cli_fastmode_writes_the_randomized_pca(tmp_path):
     got = np.array([r_[1] for r_ in rows]).T
     assert np.max(np.abs(got - v)) <= 1.01e-4 + 1e-6
     assert [r_[0] for r_ in rows] == P.ids()[0]
+
+
+@pytest.mark.gpu
+def test_cli_population_statistics_match_the_reference_log(golden, tmp_path):
+    """POPGEN's example (par.example over the CONVERTF genotypes, here read from their
+    .bed conversion): the anova p-value of each eigenvector, the population-difference
+    chi-square and its p-value, the kurtosis lines and the population sizes printed by
+    the reference (POPGEN/example.log); and the Tracy-Widom block, whose statistics
+    the tests of stats.c pin against the reference's twstats."""
+    _copy_example(golden, tmp_path)
+    write_par(tmp_path / "par", genotypename="example.bed", snpname="example.bim",
+              indivname="example.fam", evecoutname="out.evec", evaloutname="out.eval",
+              altnormstyle="NO", numoutevec=2, familynames="NO", grmoutname="grm")
+    print("[gpu-case] cli population statistics (POPGEN example)", flush=True)
+    out = run_cli(str(tmp_path / "par"), tmp_path).stdout
+    log = open(os.path.join(golden, "popgen", "example.log")).read()
+
+    def grab(text, pat):
+        return re.findall(pat, text, re.M)
+    for pat in (r"^\s+eigenvector_\d_Case_Control_\s+(\S+)",
+                r"^popdifference:\s+Case\s+Control\s+(\S+)\s+(\S+)\s+(\d+)\s+(\d+)",
+                r"^ eigenvector\s+(\d)\s+(\S+)\s+(\S+)$",
+                r"^population:\s+(\d+)\s+(\S+)\s+(\d+)$"):
+        want = grab(log, pat)
+        assert want, pat
+        assert grab(out, pat) == want, pat
+    assert "## Tracy-Widom statistics: rows: 5  cols: 7" in out
+    assert "## end of Fst statistics between populations" in out
