@@ -86,7 +86,7 @@ extern "C" void eig_close(eig_ctx *ctx) {
                     &ctx->levbf,  &ctx->live,    &ctx->xtx,        &ctx->part,     &ctx->ework,
                     &ctx->evecs_dev, &ctx->ffvecs, &ctx->fxscal,   &ctx->projw,    &ctx->proj_out,
                     &ctx->use_dev, &ctx->ignore_dev, &ctx->scratch, &ctx->scratch2,
-                    &ctx->selg,   &ctx->fill};
+                    &ctx->selg,   &ctx->fill,   &ctx->evall};
   for (DevBuf *b : bufs) free_buf(*b);
   if (ctx->hostpin.p) hipHostFree(ctx->hostpin.p);
   hipEventDestroy(ctx->ev0);

@@ -64,6 +64,7 @@ struct eig_ctx {
   // eigensolver
   DevBuf ework;
   DevBuf evecs_dev;  // nvec x nrows of the last eig_eigh_xtx
+  DevBuf evall;      // all-vector eigensolves: every batch's tridiagonal-basis vectors
   int32_t last_nvec = 0;
 
   // loadings / projection

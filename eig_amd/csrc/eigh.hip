@@ -780,16 +780,25 @@ __device__ inline double rcp_nr(double a) {
 }
 
 constexpr int kInvB = 16;  // recurrence steps per batch of loads
-__global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec) {
+// Inverse iteration for the eigenvalues j0 .. j0+nvec-1 (descending order) of the
+// tridiagonal, one lane per vector.  flags select the stages of this launch: kInvFactor
+// (the pivoted LU of T - lambda I), kInvStart (start vectors), kInvSolve and kInvGs (per
+// iteration: the two triangular solves, then Gram-Schmidt against the batch's earlier
+// vectors of the same cluster and normalisation), for `iters` iterations.  A single
+// batch runs all stages in one launch; longer runs of vectors go in batches with the
+// Gram-Schmidt against earlier batches between a launch's solves and its GS.
+enum { kInvFactor = 1, kInvStart = 2, kInvSolve = 4, kInvGs = 8, kInvAll = 15 };
+__global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec, int32_t j0,
+                                               int32_t flags, int32_t iters) {
   __shared__ double red[4];
   __shared__ double dots[2];
   const double tnorm = ws.sc[S_TNORM];
   const double eps = 2.220446049250313e-16;
   const double tiny = eps * tnorm;
   const int tid = threadIdx.x;
-  if (tid < nvec) {
+  if ((flags & kInvFactor) && tid < nvec) {
     const int kv = tid;
-    const double lam = ws.eval[n - 1 - kv];
+    const double lam = ws.eval[n - 1 - (j0 + kv)];
     double *u0 = ws.lu + (int64_t)kv * 5 * n, *u1 = u0 + n, *u2 = u1 + n, *ml = u2 + n,
            *pv = ml + n;  // u0 holds pivot reciprocals
     double acur = ws.d[0] - lam, bcur = (n > 1) ? ws.e[0] : 0.0;
@@ -840,17 +849,19 @@ __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec
     u1[n - 1] = 0.0; u2[n - 1] = 0.0; ml[n - 1] = 0.0; pv[n - 1] = 0.0;
   }
   // start vectors (a fixed hash of index and vector), filled by the whole block
-  for (int kv = 0; kv < nvec; ++kv) {
-    double *x = ws.X + (int64_t)kv * n;
-    for (int32_t i = tid; i < n; i += 256) {
-      uint32_t h = (uint32_t)(i + 1) * 2654435761u ^ (uint32_t)(kv + 7) * 40503u;
-      h ^= h >> 15; h *= 2246822519u; h ^= h >> 13;
-      x[i] = 0.5 + (double)(h & 0xFFFFFF) / 16777216.0;
+  if (flags & kInvStart)
+    for (int kv = 0; kv < nvec; ++kv) {
+      double *x = ws.X + (int64_t)kv * n;
+      const int kg = j0 + kv;
+      for (int32_t i = tid; i < n; i += 256) {
+        uint32_t h = (uint32_t)(i + 1) * 2654435761u ^ (uint32_t)(kg + 7) * 40503u;
+        h ^= h >> 15; h *= 2246822519u; h ^= h >> 13;
+        x[i] = 0.5 + (double)(h & 0xFFFFFF) / 16777216.0;
+      }
     }
-  }
   __syncthreads();
-  for (int it = 0; it < 3; ++it) {
-    if (tid < nvec) {
+  for (int it = 0; it < iters; ++it) {
+    if ((flags & kInvSolve) && tid < nvec) {
       const int kv = tid;
       const double *u0 = ws.lu + (int64_t)kv * 5 * n, *u1 = u0 + n, *u2 = u1 + n, *ml = u2 + n,
                    *pv = ml + n;
@@ -935,11 +946,12 @@ __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec
       }
     }
     __syncthreads();
+    if (!(flags & kInvGs)) continue;
     for (int kv = 0; kv < nvec; ++kv) {
       double *x = ws.X + (int64_t)kv * n;
-      const double lam = ws.eval[n - 1 - kv];
+      const double lam = ws.eval[n - 1 - (j0 + kv)];
       for (int pv2 = 0; pv2 < kv; ++pv2) {
-        const double lp = ws.eval[n - 1 - pv2];
+        const double lp = ws.eval[n - 1 - (j0 + pv2)];
         if (fabs(lam - lp) > 1e-3 * tnorm) continue;
         const double *p = ws.X + (int64_t)pv2 * n;
         double s = 0.0;
@@ -1204,8 +1216,9 @@ static TridiagGraph g_graph[16];  // per device
 int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, double *evals,
                  double *evecs, bool internal) {
   if (n <= 0) return eig_set_err(ctx, EIG_E_ARG, "eigh: n <= 0");
-  if (nvec < 0 || nvec > kMaxVec || nvec > n)
-    return eig_set_err(ctx, EIG_E_ARG, "eigh: nvec must be in [0, min(n, %d)]", kMaxVec);
+  if (nvec < 0 || nvec > n) return eig_set_err(ctx, EIG_E_ARG, "eigh: nvec must be in [0, n]");
+  if (internal && nvec > kMaxVec)
+    return eig_set_err(ctx, EIG_E_ARG, "eigh: internal calls take at most %d vectors", kMaxVec);
   // an internal call leaves the caller's stage timing (same events) running
   auto tbeg = [&]() { if (!internal) eig_time_begin(ctx); };
   auto tend = [&](const char *what) { if (!internal) eig_time_end(ctx, what); };
@@ -1300,29 +1313,81 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
   }
   EIG_HIP(ctx, hipGetLastError());
   tend("bisect");
-  // ---- eigenvectors
-  tbeg();
-  if (nvec > 0) {
-    hipLaunchKernelGGL(k_invit, dim3(1), dim3(256), 0, st, ws, n, nvec);
-    tend("invit");
-    tbeg();
-    const int32_t last = n - 2;  // last reflector row
+  // ---- eigenvectors: one batch of at most kMaxVec (inverse iteration, back-transform,
+  // sign rule), or, for more vectors, batches of kMaxVec in eigenvalue order with
+  // each iteration's vectors orthogonalised against the earlier batches' vectors of
+  // the same cluster (classical Gram-Schmidt, twice, as FP64 MFMA products) between
+  // the solves and the batch's own Gram-Schmidt
+  const int32_t last = n - 2;  // last reflector row
+  const int32_t p_last = last >= 0 ? last / kNB : -1;
+  auto back_transform = [&](int32_t nb) -> int {
     if (last >= 0) {
-      const int32_t p_last = last / kNB;
-      hipLaunchKernelGGL(k_bt_prep, dim3((unsigned)(p_last + 1)), dim3(256), 0, st, ws, n, last);
       for (int32_t p = p_last; p >= 0; --p) {
         const int32_t c0 = p * kNB;
         const unsigned g = (unsigned)((n - c0 - 1 + kBtRows - 1) / kBtRows);
-        hipLaunchKernelGGL(k_bt_step, dim3(g), dim3(256), 0, st, ws, n, p, last, nvec,
+        hipLaunchKernelGGL(k_bt_step, dim3(g), dim3(256), 0, st, ws, n, p, last, nb,
                            (int32_t)(p < p_last));
-        hipLaunchKernelGGL(k_bt_reduce, dim3(1), dim3(1024), 0, st, ws, p, last, nvec,
-                           (int32_t)g);
+        hipLaunchKernelGGL(k_bt_reduce, dim3(1), dim3(1024), 0, st, ws, p, last, nb, (int32_t)g);
       }
       hipLaunchKernelGGL(k_bt_apply0, dim3((unsigned)((n - 1 + 255) / 256)), dim3(256), 0, st, ws,
-                         n, last, nvec);
+                         n, last, nb);
     }
-    hipLaunchKernelGGL(k_finish_vec, dim3((unsigned)nvec), dim3(256), 0, st, ws, n);
+    hipLaunchKernelGGL(k_finish_vec, dim3((unsigned)nb), dim3(256), 0, st, ws, n);
     EIG_HIP(ctx, hipGetLastError());
+    return EIG_OK;
+  };
+  tbeg();
+  if (nvec > 0 && nvec <= kMaxVec) {
+    hipLaunchKernelGGL(k_invit, dim3(1), dim3(256), 0, st, ws, n, nvec, 0, (int32_t)kInvAll, 3);
+    tend("invit");
+    tbeg();
+    if (last >= 0)
+      hipLaunchKernelGGL(k_bt_prep, dim3((unsigned)(p_last + 1)), dim3(256), 0, st, ws, n, last);
+    EIG_TRY(back_transform(nvec));
+  } else if (nvec > kMaxVec) {
+    // the T-basis vectors of all batches (vector j at j*N) and the CGS coefficients
+    EIG_TRY(eig_ensure(ctx, ctx->evall, sizeof(double) * ((size_t)N * nvec + (size_t)kMaxVec * nvec)));
+    double *XT = (double *)ctx->evall.p, *Cc = XT + (size_t)N * nvec;
+    std::vector<double> ev(N);
+    double sc[S_COUNT];
+    EIG_HIP(ctx, hipMemcpyAsync(ev.data(), ws.eval, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+    EIG_HIP(ctx, hipMemcpyAsync(sc, ws.sc, sizeof(sc), hipMemcpyDeviceToHost, st));
+    EIG_HIP(ctx, hipStreamSynchronize(st));
+    const double tol = 1e-3 * sc[S_TNORM];  // the cluster criterion of k_invit
+    if (last >= 0)
+      hipLaunchKernelGGL(k_bt_prep, dim3((unsigned)(p_last + 1)), dim3(256), 0, st, ws, n, last);
+    for (int32_t j0 = 0; j0 < nvec; j0 += kMaxVec) {
+      const int32_t nb = std::min<int32_t>(kMaxVec, nvec - j0);
+      int32_t c0 = j0;  // earlier vectors within tol of eigenvalue j0 (descending order)
+      while (c0 > 0 && std::fabs(ev[N - 1 - (c0 - 1)] - ev[N - 1 - j0]) <= tol) --c0;
+      const int32_t nc = j0 - c0;
+      for (int it = 0; it < 3; ++it) {
+        const int32_t f = (it == 0 ? (kInvFactor | kInvStart) : 0) | kInvSolve;
+        hipLaunchKernelGGL(k_invit, dim3(1), dim3(256), 0, st, ws, n, nb, j0, f, 1);
+        for (int rep = 0; rep < 2 && nc > 0; ++rep) {
+          const double *P = XT + (size_t)c0 * N;
+          EIG_TRY(eig_gemm_f64(ctx, st, true, false, nc, nb, N, 1.0, P, N, 0, ws.X, N, 0, 0.0, Cc,
+                               nc, 0, 1));
+          EIG_TRY(eig_gemm_f64(ctx, st, false, false, N, nb, nc, -1.0, P, N, 0, Cc, nc, 0, 1.0,
+                               ws.X, N, 0, 1));
+        }
+        hipLaunchKernelGGL(k_invit, dim3(1), dim3(256), 0, st, ws, n, nb, j0, (int32_t)kInvGs, 1);
+      }
+      EIG_HIP(ctx, hipMemcpyAsync(XT + (size_t)j0 * N, ws.X, sizeof(double) * N * nb,
+                                  hipMemcpyDeviceToDevice, st));
+      EIG_TRY(back_transform(nb));
+      if (evecs)
+        EIG_HIP(ctx, hipMemcpyAsync(evecs + (size_t)j0 * N, ws.X, sizeof(double) * N * nb,
+                                    hipMemcpyDeviceToHost, st));
+      if (!internal && j0 == 0) {
+        EIG_TRY(eig_ensure(ctx, ctx->evecs_dev, sizeof(double) * N * kMaxVec));
+        EIG_HIP(ctx, hipMemcpyAsync(ctx->evecs_dev.p, ws.X, sizeof(double) * N * kMaxVec,
+                                    hipMemcpyDeviceToDevice, st));
+      }
+    }
+    EIG_HIP(ctx, hipStreamSynchronize(st));
+    tend("invit");
+    tbeg();
   } else {
     tend("invit");
     tbeg();
@@ -1332,13 +1397,14 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
   int32_t hsync[2] = {0, 0};
   EIG_HIP(ctx, hipMemcpyAsync(hsync, ws.sync, sizeof(hsync), hipMemcpyDeviceToHost, st));
   EIG_HIP(ctx, hipMemcpyAsync(asc.data(), ws.eval, sizeof(double) * N, hipMemcpyDeviceToHost, st));
-  if (nvec > 0 && evecs)
-    EIG_HIP(ctx, hipMemcpyAsync(evecs, ws.X, sizeof(double) * N * nvec, hipMemcpyDeviceToHost, st));
-  if (!internal) {
-    EIG_TRY(eig_ensure(ctx, ctx->evecs_dev, sizeof(double) * N * std::max(nvec, 1)));
-    if (nvec > 0)
+  if (nvec > 0 && nvec <= kMaxVec) {
+    if (evecs)
+      EIG_HIP(ctx, hipMemcpyAsync(evecs, ws.X, sizeof(double) * N * nvec, hipMemcpyDeviceToHost, st));
+    if (!internal) {
+      EIG_TRY(eig_ensure(ctx, ctx->evecs_dev, sizeof(double) * N * std::max(nvec, 1)));
       EIG_HIP(ctx, hipMemcpyAsync(ctx->evecs_dev.p, ws.X, sizeof(double) * N * nvec,
                                   hipMemcpyDeviceToDevice, st));
+    }
   }
   EIG_HIP(ctx, hipStreamSynchronize(st));
   if (hsync[1] != 0)

@@ -295,6 +295,9 @@ int main(int argc, char **argv) {
   for (int i = 0; i < pp.n; ++i) printf("%s %s\n", pp.key[i], pp.val[i]);
   if (!S.genotypename || !S.snpname || !S.indivname)
     die("genotypename, snpname and indivname are required");
+  /* the loadings and the projection keep K x K normal equations in registers */
+  if (S.numeigs > 16 && !S.fastmode)
+    die("numoutevec: %d -- this build writes at most 16 output eigenvectors", S.numeigs);
   if (S.xchrom == S.numchrom + 1) S.noxdata = 0;
   if (S.ldlimit <= 0 || S.ldposlimit <= 0) S.ldregress = 0; /* smartpca.c:743-744 */
   /* usepopsformissing, ldregress and SNP weights take the non-lookup column path
@@ -557,8 +560,10 @@ int main(int argc, char **argv) {
   }
   int numoutiter = S.numoutliter >= 1 ? S.numoutliter + 1 : 1;
   int omode = S.numoutliter >= 1 ? S.outliermode : 2;
-  int nvec_pass = S.numeigs > S.numoutleigs ? S.numeigs : S.numoutleigs;
-  if (nvec_pass > 64) nvec_pass = 64;
+  /* the eigenvectors a pass needs: the output axes and, when outliers are removed,
+     the outlier axes; the device computes all of them, so none is ever dropped */
+  int nvec_pass = S.numeigs;
+  if (S.numoutliter >= 1 && S.numoutleigs > nvec_pass) nvec_pass = S.numoutleigs;
   double t_xtx = 0, t_eig = 0;
   for (int outliter = 1; outliter <= numoutiter; ++outliter) {
     nrows = 0;

@@ -261,3 +261,38 @@ def test_eigh_rejects_non_finite_input_before_any_launch(eng):
             eng.eigh(a, 2)
     w, v = eng.eigh(np.diag([3.0, 2.0, 1.0]), 1)  # the context still works
     assert np.allclose(w, [3.0, 2.0, 1.0]) and np.allclose(np.abs(v[0]), [1.0, 0.0, 0.0])
+
+
+@pytest.mark.parametrize("n,kind", [(300, "random"), (257, "clustered"), (700, "structure")])
+def test_eigh_all_vectors_in_batches(eng, n, kind):
+    """eigvecs' contract (eigsubs.c:39-55): every eigenvector.  Past 64 vectors the
+    eigensolver runs batches of 64 and orthogonalises each against the earlier batches'
+    vectors of its cluster; checked against LAPACK dspev: eigenvalues, orthonormality of
+    all n vectors, residuals, and the vectors of well separated eigenvalues."""
+    rng = np.random.default_rng(n)
+    if kind == "random":
+        a = rng.standard_normal((n, n))
+        a = a @ a.T / n
+    else:
+        u = np.linalg.qr(rng.standard_normal((n, n)))[0]
+        if kind == "clustered":  # a 150-fold eigenvalue spanning three batches
+            lam = np.concatenate([np.linspace(40, 5, 50), np.ones(150), np.linspace(0.5, 0.01, n - 200)])
+        else:  # few large eigenvalues over a Marchenko-Pastur-like bulk, as an XTX has
+            lam = np.concatenate([[60, 30, 12], rng.uniform(0.8, 1.2, n - 3)])
+        a = (u * lam) @ u.T
+        a = (a + a.T) / 2
+    _say(f"eigh all vectors n={n} {kind}")
+    w, v = eng.eigh(a, n)
+    wr, vr = O.dspev_lower(a)
+    lam_max = abs(wr[0])
+    assert np.max(np.abs(w - wr)) <= 1e-10 * lam_max
+    orth = np.max(np.abs(v @ v.T - np.eye(n)))
+    res = np.max(np.abs(a @ v.T - v.T * w))
+    _say(f"all vectors n={n} {kind}: |V^T V - I| {orth:.2e}, residual {res:.2e}")
+    assert orth <= 1e-10
+    assert res <= 1e-9 * lam_max
+    vr = O.sign_rule(vr)
+    for j in range(n):
+        sep = min(abs(wr[j] - wr[k]) for k in range(n) if k != j)
+        if sep > 1e-3 * lam_max:
+            assert np.max(np.abs(v[j] - vr[j])) < 1e-8 / min(1.0, sep), j
