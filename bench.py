@@ -52,14 +52,14 @@ def cpu_baseline(eng, panel, nthreads):
     full workload (XTX linear in M, dspev cubic in N)."""
     from oracle import oracle as O
     N, M = panel.nind, panel.nsnp
-    ms = 4000
+    ms = 8000
     rows = eng.fetch_packed(0, ms)
     xindex = np.arange(N, dtype=np.int32)
     t0 = time.perf_counter()
     st = O.snpstats(rows, xindex)
     O.xtx_lookup(rows, xindex, st["levels"], st["status"], nthreads=nthreads)
     t_x = time.perf_counter() - t0
-    ne = 1500
+    ne = 2500
     rng = np.random.default_rng(1)
     a = rng.standard_normal((ne, ne))
     a = a @ a.T
@@ -68,8 +68,29 @@ def cpu_baseline(eng, panel, nthreads):
     t_e = time.perf_counter() - t0
     est = t_x * (M / ms) + t_e * (N / ne) ** 3
     return dict(value=round(est, 2), unit="s", cores=nthreads, kind="port",
-                sample=f"lookup XTX on {ms} of {M} SNPs x {N} samples ({t_x:.2f}s, x{M // ms}) + "
+                sample=f"lookup XTX on {ms} of {M} SNPs x {N} samples ({t_x:.2f}s, x{M / ms:g}) + "
                        f"LAPACK dspev at n={ne} ({t_e:.2f}s, x(N/n)^3); per pass")
+
+
+def cpu_cfg1_full(nthreads, runs=3):
+    """The reference CPU pass in full at N=1,000 x M=50,000 (3 populations, 1% missing):
+    column statistics, partial-sum lookup XTX, trace scaling and LAPACK dspev; median
+    of `runs`, in seconds."""
+    from eig_amd import synth
+    from oracle import oracle as O
+    P = synth.config_panel(1)
+    rows = P.packed()
+    xindex = np.arange(P.nind, dtype=np.int32)
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        st = O.snpstats(rows, xindex)
+        full, _ = O.finalize_xtx(O.xtx_lookup(rows, xindex, st["levels"], st["status"],
+                                              nthreads=nthreads), P.nind)
+        O.dspev_lower(full)
+        times.append(time.perf_counter() - t0)
+    return dict(s=round(float(np.median(times)), 3), runs=runs, cores=nthreads,
+                N=P.nind, M=P.nsnp, lapack="numpy's bundled scipy_openblas64 dspev")
 
 
 def serve_leg(eng, P, vec, seconds, rate=1000.0, pool_n=4096):
@@ -202,6 +223,70 @@ def fastmode_leg(args):
     print(json.dumps(out))
 
 
+def ridoutlier(evecs, nle, thresh):
+    """The z-score outlier rule (smartsubs.c:18-93, outliermode 0): sample i is removed
+    when |v_i - mean(v)| / sqrt(mean((v - mean)^2)) exceeds thresh on any of the top
+    nle eigenvectors."""
+    bad = np.zeros(evecs.shape[1], dtype=bool)
+    for e in range(nle):
+        v = evecs[e]
+        w = v - v.mean()
+        bad |= np.abs(w / np.sqrt(np.dot(w, w) / len(w))) > thresh
+    return np.nonzero(bad)[0]
+
+
+def cfg3_leg(args):
+    """The 25-population admixed panel at N=20,000 x M=1,000,000 (5% admixed, 3% missing)
+    through smartpca's outlier loop (5 iterations, 6 sigma, 10 outlier eigenvectors),
+    generated in HBM: each form's passes (statistics, XTX, eigensolve) timed, the
+    removals, and the final top-10 eigenvalues of the two forms compared."""
+    import eig_amd
+    from eig_amd import synth
+    P = synth.config_panel(3, nsnp=None if args.nsnp == 600000 else args.nsnp, scale=args.scale)
+    N, M = P.nind, P.nsnp
+    log(f"cfg3: generating N={N} M={M} in HBM")
+    eng = eig_amd.Engine(0)
+    P.to_device(eng)
+    out = {"metric": "smartpca outlier loop s (5 iterations, 6 sigma, 10 eigenvectors)",
+           "unit": "s", "n_gpus": 1,
+           "config": {"workload": "25 Balding-Nichols pops x 800, 5% admixed (Dirichlet over 3), "
+                                  "3% missing", "N": N, "M": M}, "forms": {}}
+    for form, name in ((eig_amd.FORM_BF16X4, "bf16x4"), (eig_amd.FORM_FP64, "fp64")):
+        keep = np.ones(N, dtype=bool)
+        dead = np.zeros(M, dtype=bool)
+        passes, removed = [], []
+        t_all = time.perf_counter()
+        for it in range(1, 7):
+            xindex = np.nonzero(keep)[0].astype(np.int32)
+            t0 = time.perf_counter()
+            eng.select(xindex)
+            st = eng.snpstats(live=(~dead).astype(np.uint8))
+            eng.xtx(form, BF16_BLOCK, want_matrix=False)
+            ev, vec = eng.eigh_xtx(10)
+            dt = time.perf_counter() - t0
+            dead |= st["status"] != 0
+            passes.append(dict(nrows=len(xindex), s=round(dt, 3), syrk_ms=eng.last_ms("syrk"),
+                               tridiag_ms=eng.last_ms("tridiag"), trace_sum=float(ev.sum())))
+            log(f"cfg3 {name} pass {it}: nrows {len(xindex)} {dt:.3f} s")
+            if it == 6:
+                break
+            bad = ridoutlier(vec, min(10, len(xindex) - 1), 6.0)
+            if len(bad) == 0:
+                break
+            removed.append([int(x) for x in xindex[bad]])
+            keep[xindex[bad]] = False
+        out["forms"][name] = dict(s=round(time.perf_counter() - t_all, 3), passes=passes,
+                                  removed=removed, top10=[float(x) for x in ev[:10]])
+    b, f = out["forms"]["bf16x4"], out["forms"]["fp64"]
+    out["same_removals"] = b["removed"] == f["removed"]
+    if out["same_removals"]:
+        out["top10_eval_relerr_bf16x4_vs_fp64"] = float(np.max(
+            np.abs(np.array(b["top10"]) - np.array(f["top10"])) / np.abs(np.array(f["top10"]))))
+    out["speedup_vs_fp64"] = round(f["s"] / b["s"], 3)
+    eng.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +300,9 @@ def main():
     ap.add_argument("--forms", default="bf16x4,bf16x3,fp64",
                     help="forms to run; anything but all three prints stage times only "
                          "(for profiler runs)")
+    ap.add_argument("--cfg3", action="store_true",
+                    help="instead of the XTX pass, run the 25-population admixed panel at "
+                         "20,000 x 1,000,000 through the outlier loop in both forms")
     ap.add_argument("--fastmode", action="store_true",
                     help="instead of the XTX pass, time fastmode (K=10, L=20, I=10) on the "
                          "100,000 x 500,000 panel in both panel-product forms")
@@ -230,6 +318,9 @@ def main():
     from eig_amd import synth
     if args.fastmode:
         fastmode_leg(args)
+        return
+    if args.cfg3:
+        cfg3_leg(args)
         return
     nvec = 10
     P = synth.config_panel(2, nsnp=args.nsnp, scale=args.scale)
@@ -279,17 +370,25 @@ def main():
         return rel, cos, allabs
     relerr, cosmin, allabs = errs(b)
     relerr3, cosmin3, allabs3 = errs(b3)
-    flops = 4.0 * N * (N + 1) * M
-    ach = flops / (b["syrk_ms"] * 1e-3) / 1e12
+    mfma_flops = 4.0 * N * (N + 1) * M   # the four BF16 products on MFMA
+    alg_flops = 3.0 * N * (N + 1) * M    # the compensated SYRK's three products
+    useful_flops = 1.0 * N * (N + 1) * M  # the XTX itself (lower triangle, 2 flop/MAC)
+    syrk_s = b["syrk_ms"] * 1e-3
     tri_bytes = 8 * (N - 1) * N * (2 * N - 1) // 6  # sum over L = 1 .. N-1 of L^2 doubles
-    # HBM bytes per launch of the headline kernel, from the two rocprofv3 --pmc runs
-    # (FETCH_SIZE, WRITE_SIZE) of this workload scaled by a 1 GiB copy
-    traffic = None
+    tri_ach = tri_bytes / (b["parts"]["tridiag"] * 1e-3) / 1e9
+    # decode + normalise: GENO rows read (rlen each) and 28 B of per-SNP statistics written
+    rlen = max(48, (2 * N + 7) // 8)
+    dec_bytes = M * rlen + 28 * M
+    dec_ms = b["select_ms"] + b["stats_ms"]
+    # HBM bytes per launch of the dominant stage's kernels, from rocprofv3 --pmc runs
+    # (FETCH_SIZE, WRITE_SIZE) of this workload scaled by a 1 GiB copy: a stored figure
+    traffic, tsrc = None, None
     tpath = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tpath) and (N, M) == (5000, 600000):
-        t = json.load(open(tpath)).get("k_syrk_bf16<true, true>")
+        t = json.load(open(tpath)).get("tridiag_pass")
         if t:
             traffic = int(t["read_bytes"] + t["write_bytes"])
+            tsrc = t["source"] + " (stored, not this run)"
     out = {
         "metric": METRIC, "value": round(b["t"], 5), "unit": "s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(b["t"] * 1e3, 3),
@@ -311,19 +410,31 @@ def main():
                                  "syrk": b["syrk_ms"], **b["parts"]},
                       "bf16x3": {"syrk": b3["syrk_ms"], **b3["parts"]},
                       "fp64": {"syrk": f["syrk_ms"], **f["parts"]}},
-        "roofline": {"bound": "mfma", "achieved": round(ach, 2), "peak": PEAK_BF16,
-                     "unit": "TFLOP/s", "frac": round(ach / PEAK_BF16, 4), "traffic": traffic,
-                     "kernel": "k_syrk_bf16<true, true> (4 products)", "algorithmic_flops": flops},
-        "fp64_syrk_roofline": {"achieved": round(N * (N + 1) * M / (f["syrk_ms"] * 1e-3) / 1e12, 2),
-                               "peak": PEAK_FP64, "unit": "TFLOP/s"},
-        # the eigensolver's largest stage against HBM: its symv streams each column's
-        # L x L trailing square once (L = N-1 .. 1), timed as a whole stage (per-column
-        # latency included)
-        "tridiag_roofline": {"bound": "hbm",
-                             "achieved": round(tri_bytes / (b["parts"]["tridiag"] * 1e-3) / 1e9, 1),
-                             "peak": PEAK_HBM, "unit": "GB/s",
-                             "frac": round(tri_bytes / (b["parts"]["tridiag"] * 1e-3) / 1e9 / PEAK_HBM, 4),
-                             "algorithmic_bytes": tri_bytes, "ms": round(b["parts"]["tridiag"], 3)},
+        # the dominant stage of the headline pass: the tridiagonalisation, one graph
+        # launch per pass (k_symv / k_wcol / k_column / k_syr2k); its symv streams each
+        # column's L x L trailing square once (L = N-1 .. 1); HIP-event time of the
+        # launch on the context stream, per-column latency included
+        "roofline": {"bound": "hbm", "achieved": round(tri_ach, 1), "peak": PEAK_HBM,
+                     "unit": "GB/s", "frac": round(tri_ach / PEAK_HBM, 4), "traffic": traffic,
+                     "traffic_source": tsrc,
+                     "kernel": "tridiagonalisation graph (k_symv<4>, k_wcol, k_column, k_syr2k)",
+                     "algorithmic_bytes": tri_bytes, "ms": round(b["parts"]["tridiag"], 3)},
+        "syrk_roofline": {"kernel": "k_syrk_bf16<true, true>", "bound": "mfma", "ms": round(b["syrk_ms"], 3),
+                          "peak": PEAK_BF16, "unit": "TFLOP/s",
+                          "mfma_achieved": round(mfma_flops / syrk_s / 1e12, 1),
+                          "mfma_frac": round(mfma_flops / syrk_s / 1e12 / PEAK_BF16, 4),
+                          "alg_3product_achieved": round(alg_flops / syrk_s / 1e12, 1),
+                          "alg_3product_frac": round(alg_flops / syrk_s / 1e12 / PEAK_BF16, 4),
+                          "useful_rate": round(useful_flops / syrk_s / 1e12, 1),
+                          "mfma_flops": mfma_flops, "alg_flops": alg_flops},
+        "fp64_syrk_roofline": {"kernel": "k_syrk_f64",
+                               "achieved": round(useful_flops / (f["syrk_ms"] * 1e-3) / 1e12, 2),
+                               "peak": PEAK_FP64, "unit": "TFLOP/s",
+                               "frac": round(useful_flops / (f["syrk_ms"] * 1e-3) / 1e12 / PEAK_FP64, 4)},
+        "decode_normalise": {"kernels": "k_select + k_snpstats", "bytes": dec_bytes,
+                             "ms": round(dec_ms, 4),
+                             "achieved_GBps": round(dec_bytes / (dec_ms * 1e-3) / 1e9, 1),
+                             "frac": round(dec_bytes / (dec_ms * 1e-3) / 1e9 / PEAK_HBM, 4)},
     }
     if not args.no_serve:
         log("serving leg")
@@ -338,6 +449,8 @@ def main():
         nthreads = min(nthreads, 16)
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(eng, P, nthreads)
+        log("cpu pass in full at N=1,000 x M=50,000")
+        out["cpu_cfg1_full"] = cpu_cfg1_full(nthreads)
     eng.close()
     print(json.dumps(out))
 

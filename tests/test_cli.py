@@ -206,24 +206,73 @@ def test_cli_matches_oracle_on_synthetic_panel(tmp_path, form):
 
 
 @pytest.mark.gpu
-def test_cli_outlier_removal_matches_oracle(tmp_path):
+@pytest.mark.parametrize("mode", [0, 1])
+def test_cli_outlier_removal_matches_oracle(tmp_path, mode):
+    """Outlier iterations with the ordinary z-score (outliermode 0) and the
+    leave-one-out z-score (outliermode 1, smartsubs.c:59-82)."""
     P, prefix = _synthetic(tmp_path, 3000, 0.15, plant=7)
     par = dict(genotypename=prefix + ".geno", snpname=prefix + ".snp",
                indivname=prefix + ".ind", evecoutname="out.evec", evaloutname="out.eval",
                outlieroutname="out.outl", numoutevec=4, numoutlieriter=3, numoutlierevec=4,
-               outliersigmathresh=6.0)
+               outliersigmathresh=6.0, outliermode=mode)
     write_par(tmp_path / "par", **par)
-    print(f"[gpu-case] cli outliers N={P.nind} M={P.nsnp}", flush=True)
+    print(f"[gpu-case] cli outliers N={P.nind} M={P.nsnp} outliermode={mode}", flush=True)
     run_cli(str(tmp_path / "par"), tmp_path)
     ref = O.smartpca({k: str(v) for k, v in par.items()}, str(tmp_path))
     got = (tmp_path / "out.outl").read_text().splitlines()
     want = ["REMOVED outlier %s iter %d evec %d sigmage %.3f pop: %s" % r for r in ref["removed"]]
     assert len(want) >= 1
     assert [g.split()[:7] for g in got] == [w.split()[:7] for w in want]
-    assert np.allclose([float(g.split()[8]) for g in got], [float(w.split()[8]) for w in want],
-                       atol=2e-3)
+    # the score's sign follows the eigenvector's sign (LAPACK's in the reference, the
+    # sign rule here): magnitudes are compared
+    assert np.allclose([abs(float(g.split()[8])) for g in got],
+                       [abs(float(w.split()[8])) for w in want], atol=2e-3)
     ids = [l.split()[0] for l in (tmp_path / "out.evec").read_text().splitlines()[1:]]
     assert P.ids()[0][7] not in ids
+
+
+@pytest.mark.gpu
+def test_cli_grm_binary_snp_weights_and_best_snps_match_oracle(tmp_path):
+    """dumpgrmbin (smartpca.c:3705-3767): float32 XTX[a][b] / (trace/nrows) over the
+    lower triangle and the SNP count as int32 in .N.bin; writesnpeigs
+    (smartpca.c:3038-3104): each axis's loadings scaled to mean square 1 over the
+    pass's SNPs, the 10 largest |weights| per axis as 'eigbestsnp' lines, then one
+    line per kept SNP.  Expected values from the oracle's XTX and loadings."""
+    P, prefix = _synthetic(tmp_path, 3000, 0.15)
+    par = dict(genotypename=prefix + ".geno", snpname=prefix + ".snp",
+               indivname=prefix + ".ind", evecoutname="out.evec", evaloutname="out.eval",
+               numoutevec=3, numoutlieriter=0, grmoutname="grm", grmbinary="YES",
+               snpweightoutname="out.snpw")
+    write_par(tmp_path / "par", **par)
+    print(f"[gpu-case] cli grm binary + snp weights N={P.nind} M={P.nsnp}", flush=True)
+    out = run_cli(str(tmp_path / "par"), tmp_path).stdout
+    ref = O.smartpca({k: str(v) for k, v in par.items()}, str(tmp_path))
+    n = len(ref["xindex"])
+    X = ref["xtx"]
+    want = (X[np.tril_indices(n)] / (np.trace(X) / n)).astype(np.float32)
+    got = np.fromfile(tmp_path / "grm.bin", dtype=np.float32)
+    assert got.shape == want.shape
+    # one float32 rounding apart at most
+    assert np.max(np.abs(got - want) / np.maximum(np.abs(want), 1e-3)) <= 2.4e-7
+    cnt = np.fromfile(tmp_path / "grm.N.bin", dtype=np.int32)
+    assert cnt.shape == want.shape and np.all(cnt == int(ref["ynumsnps"]))
+    # SNP weights
+    snp = O.read_snp(prefix + ".snp")
+    W = ref["ffvecs"] * np.sqrt(ref["ffvecs"].shape[1] / np.sum(ref["ffvecs"] ** 2, axis=1))[:, None]
+    keep = ref["snp_ignore_final"] == 0
+    cols = ref["cols"]
+    lines = (tmp_path / "out.snpw").read_text().splitlines()
+    assert [l.split()[0] for l in lines] == [snp["ids"][m] for m in cols[keep]]
+    assert [int(l.split()[1]) for l in lines] == [int(snp["chrom"][m]) for m in cols[keep]]
+    gw = np.array([[float(x) for x in l.split()[3:]] for l in lines])
+    assert np.max(np.abs(gw - W[:, keep].T)) <= 1.01e-3
+    best = re.findall(r"^eigbestsnp\s+(\d+)\s+(\S+)\s+\d+\s+\d+\s+(\S+)$", out, re.M)
+    assert len(best) == 30
+    for j in range(3):
+        a = np.abs(W[j]) * keep - (~keep)
+        top = np.argsort(-a, kind="stable")[:10]
+        assert [b[1] for b in best if int(b[0]) == j + 1] == [snp["ids"][cols[t]] for t in top]
+        assert np.allclose([float(b[2]) for b in best if int(b[0]) == j + 1], a[top], atol=1.01e-3)
 
 
 @pytest.mark.gpu
@@ -259,7 +308,7 @@ def test_cli_population_statistics_match_the_reference_log(golden, tmp_path):
     """POPGEN's example (par.example over the CONVERTF genotypes, here read from their
     .bed conversion): the anova p-value of each eigenvector, the population-difference
     chi-square and its p-value, the kurtosis lines and the population sizes printed by
-    the reference (POPGEN/example.log); and the Tracy-Widom block, whose statistics
+    the reference (POPGEN/example.log, kept as example_log.txt); and the Tracy-Widom block, whose statistics
     the tests of stats.c pin against the reference's twstats."""
     _copy_example(golden, tmp_path)
     write_par(tmp_path / "par", genotypename="example.bed", snpname="example.bim",
@@ -267,7 +316,7 @@ def test_cli_population_statistics_match_the_reference_log(golden, tmp_path):
               altnormstyle="NO", numoutevec=2, familynames="NO", grmoutname="grm")
     print("[gpu-case] cli population statistics (POPGEN example)", flush=True)
     out = run_cli(str(tmp_path / "par"), tmp_path).stdout
-    log = open(os.path.join(golden, "popgen", "example.log")).read()
+    log = open(os.path.join(golden, "popgen", "example_log.txt")).read()
 
     def grab(text, pat):
         return re.findall(pat, text, re.M)
@@ -280,3 +329,55 @@ def test_cli_population_statistics_match_the_reference_log(golden, tmp_path):
         assert grab(out, pat) == want, pat
     assert "## Tracy-Widom statistics: rows: 5  cols: 7" in out
     assert "## end of Fst statistics between populations" in out
+
+
+@pytest.mark.gpu
+def test_cli_deleted_snp_log_follows_the_reference(tmp_path):
+    """deletsnpoutname (logdeletedsnp, mcio.c:4893-4900; smartpca.c:801-852, 1131-1139):
+    chromosome filters first, then the position filters each on its own line (also for
+    SNPs already dropped), then the no-data and the keep-rule drops, in the reference's
+    '%-40s  %-40s' format."""
+    from eig_amd import synth
+    P = synth.config_panel(1, nsnp=1200, scale=0.1)
+    prefix = str(tmp_path / "del")
+    synth.write_files(P, prefix)
+    snp_lines = open(prefix + ".snp").read().splitlines()
+    # chromosome 23 (X), 0 and 30 on a few SNPs; positions as written by write_files
+    for m, ch in ((3, 23), (4, 0), (5, 30)):
+        f = snp_lines[m].split()
+        f[1] = str(ch)
+        snp_lines[m] = " ".join(f)
+    open(prefix + ".snp", "w").write("\n".join(snp_lines) + "\n")
+    # a monomorphic SNP on chromosome 2 (dropped by the keep rule); order within the
+    # chromosome is unchanged, so the packed order check still holds
+    raw = bytearray(open(prefix + ".geno", "rb").read())
+    rlen = max(48, (2 * P.nind + 7) // 8)
+    mono = [m for m in range(P.nsnp) if snp_lines[m].split()[1] == "2"][7]
+    raw[rlen * (mono + 1): rlen * (mono + 2)] = bytes(rlen)  # all code 0
+    open(prefix + ".geno", "wb").write(bytes(raw))
+    write_par(tmp_path / "par", genotypename=prefix + ".geno", snpname=prefix + ".snp",
+              indivname=prefix + ".ind", evecoutname="out.evec", numoutevec=2,
+              numoutlieriter=0, chrom=2, lopos=1000 + 100 * (mono - 20),
+              hipos=1000 + 100 * (mono + 20), deletsnpoutname="deleted.txt", hashcheck="NO",
+              pordercheck="NO")
+    print("[gpu-case] cli deleted-SNP log", flush=True)
+    run_cli(str(tmp_path / "par"), tmp_path)
+    ids = [l.split()[0] for l in snp_lines]
+    chrom = [int(l.split()[1]) for l in snp_lines]
+    pos = [int(l.split()[3]) for l in snp_lines]
+    want = []
+    for m in range(P.nsnp):
+        why = ("chrom-X" if chrom[m] == 23 else "chrom-0" if chrom[m] == 0
+               else "chrom-big" if chrom[m] > 23 else None)
+        if why:
+            want.append((ids[m], why))
+    for m in range(P.nsnp):
+        if chrom[m] != 2:
+            want.append((ids[m], "not-chrom"))
+        if pos[m] < 1000 + 100 * (mono - 20):
+            want.append((ids[m], "lopos"))
+        if pos[m] > 1000 + 100 * (mono + 20):
+            want.append((ids[m], "hipos"))
+    want.append((ids[mono], "minallelecnt"))
+    got = (tmp_path / "deleted.txt").read_text().splitlines()
+    assert got == ["%-40s  %-40s" % w for w in want]

@@ -88,7 +88,7 @@ def test_xtx_normal_matches_oracle(eng, case):
         Z, ynum = O.ldregress_columns(nc["cols"], kept, chrom, ppos, gpos, ld, ldposlimit=lposl)
         ndrop = int(np.sum(~np.any(Z != 0, axis=1)))
         _say(f"{case}: {ndrop} of {len(kept)} SNPs dropped, ynumsnps {ynum:.6f} (gpu {yn:.6f})")
-        assert ndrop >= len(kept) // 10
+        assert ndrop >= len(kept) // 20  # most of the exact duplicates
     else:
         Z, ynum = nc["cols"][kept], float(len(kept))
     ref, yr = O.dense_xtx(Z)

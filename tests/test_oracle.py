@@ -189,6 +189,33 @@ def test_ridoutlier_planted():
     assert len(O.ridoutlier(ev, 3, 6.0, 2)[0]) == 0        # mode 2 off
 
 
+def test_ridoutlier_leave_one_out_hand_worked():
+    """outliermode 1 (smartsubs.c:59-82): sample j's z-score uses the mean of the other
+    n-1 values and their squared deviations summed over those n-1 but divided by n.
+    Row 0: 200 values +-1 (mean 0) and 6.0 at index 50, so z_50 = 6 sqrt(201/200) =
+    6.01497 > 6; a divisor of n-1 would give exactly 6.0 (not flagged), and the
+    ordinary z of outliermode 0 gives 5.51 (not flagged either)."""
+    n = 201
+    ev = np.zeros((2, n))
+    ev[0] = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
+    ev[0, 50:] = np.roll(ev[0, 50:], 1)  # keep 100 of each sign around index 50
+    ev[0, 50] = 6.0
+    assert np.sum(np.delete(ev[0], 50)) == 0.0
+    # row 1: a smaller outlier at 120 on top of a spread-out background
+    ev[1] = np.linspace(-1.0, 1.0, n)
+    ev[1, 120] = -9.0
+    bad, vno, sc = O.ridoutlier(ev, 2, 6.0, 1)
+    assert list(bad) == [50, 120] and list(vno) == [0, 1]
+    assert sc[0] == pytest.approx(6.0 * np.sqrt(201 / 200), rel=1e-14)
+    rest = np.delete(ev[1], 120)
+    mu = rest.mean()
+    z = (-9.0 - mu) / np.sqrt(np.sum((rest - mu) ** 2) / n)
+    assert sc[1] == pytest.approx(z, rel=1e-13)
+    # every other sample stays well inside the threshold; mode 0 flags only row 1's
+    assert len(O.ridoutlier(ev, 1, 6.0, 0)[0]) == 0
+    assert list(O.ridoutlier(ev, 2, 6.0, 0)[0]) == [120]
+
+
 def _lsq_setup(rng, M=400, N=60, K=4, miss=0.0):
     g = _rand_geno(rng, M, N, miss)
     rows = O.pack_genotypes(g)
