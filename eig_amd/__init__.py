@@ -51,6 +51,7 @@ SIGNATURES = {
     "eig_loadings": (C.c_int, [_vp, _f64p, C.c_int32, _vp, _vp, _vp]),
     "eig_lsqproject": (C.c_int, [_vp, _i32p, _f64p, _f64p, _i32p]),
     "eig_lsqproject_fill": (C.c_int, [_vp, _i32p, _i32p, C.c_int32, _f64p, _f64p, _i32p]),
+    "eig_lsqproject_noalloc": (C.c_int, [_vp, _i32p, _f64p, _f64p, _i32p]),
     "eig_fpca": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, _vp, _f64p, _vp]),
     "eig_fpca_form": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, _vp, C.c_int32,
                                 _f64p, _vp]),

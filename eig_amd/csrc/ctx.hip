@@ -51,6 +51,13 @@ static void free_buf(DevBuf &b) {
   b.bytes = 0;
 }
 
+// Workspace of eig_lsqproject_noalloc, the entry point for callers that cannot
+// allocate device memory: one 64 MiB array per GPU, held until the process ends.
+// Host code uses only the device address from hipGetSymbolAddress (eig_open stores
+// it in the context); calls that overlap on one GPU share it and race.
+constexpr size_t kNoAllocBytes = (size_t)64 << 20;
+static __device__ __align__(256) char scratch[kNoAllocBytes];
+
 extern "C" int eig_open(int device, eig_ctx **out) {
   if (!out) return EIG_E_ARG;
   *out = nullptr;
@@ -65,9 +72,14 @@ extern "C" int eig_open(int device, eig_ctx **out) {
             prop.gcnArchName);
     return EIG_E_HIP;
   }
+  void *ws = nullptr;
+  if (hipGetSymbolAddress(&ws, HIP_SYMBOL(scratch)) != hipSuccess || ws == nullptr)
+    return EIG_E_HIP;
   eig_ctx *c = new eig_ctx();
   c->device = device;
   c->num_cu = prop.multiProcessorCount;
+  c->noalloc = ws;
+  c->noalloc_bytes = kNoAllocBytes;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     delete c;

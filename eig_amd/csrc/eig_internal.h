@@ -70,6 +70,10 @@ struct eig_ctx {
   // loadings / projection
   DevBuf ffvecs, fxscal, projw, proj_out, use_dev, ignore_dev;
   int32_t last_K = 0;
+  // device address of this GPU's static workspace of eig_lsqproject_noalloc (set by
+  // eig_open through hipGetSymbolAddress; never the array's host-side name)
+  void *noalloc = nullptr;
+  size_t noalloc_bytes = 0;
   bool have_load = false;
 
   // generic scratch

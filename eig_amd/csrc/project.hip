@@ -452,3 +452,105 @@ extern "C" int eig_lsqproject_fill(eig_ctx *ctx, const int32_t *use, const int32
   if (!itype) return eig_set_err(ctx, EIG_E_ARG, "eig_lsqproject_fill: itype is required");
   return lsqproject(ctx, use, itype, npop, acoeffs, bcoeffs, status);
 }
+
+// ---------------------------------------------------------------- allocation-free lsqproject
+namespace {
+__global__ void k_zero(double *__restrict__ p, int64_t n) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e < n) p[e] = 0.0;
+}
+
+// acc[q][p] += part[c][q][p] for the group's chunks c in order: the same sequence of
+// additions as k_lsq_solve's loop over all chunks
+__global__ void k_lsq_accum(double *__restrict__ acc, const double *__restrict__ part,
+                            int64_t nchunk, int64_t per) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= per) return;
+  double v = acc[e];
+  for (int64_t c = 0; c < nchunk; ++c) v += part[c * per + e];
+  acc[e] = v;
+}
+
+// the stale-answer rule of the reference, in individual order, one thread per axis j:
+// outputs of individuals with status 1 or 3 keep the caller's values
+__global__ void k_lsq_stale(const double *__restrict__ ans, const double *__restrict__ rhs,
+                            const int32_t *__restrict__ st, int32_t nind, int32_t K,
+                            double *__restrict__ A, double *__restrict__ B) {
+  const int j = threadIdx.x;
+  if (j >= K) return;
+  double last = 0.0;
+  for (int32_t q = 0; q < nind; ++q) {
+    const int32_t s = st[q];
+    if (s == 0) last = ans[(int64_t)j * nind + q];
+    if (s == 0 || s == 2) {
+      if (A) A[(int64_t)j * nind + q] = last;
+      if (B) B[(int64_t)j * nind + q] = rhs[(int64_t)j * nind + q];
+    }
+  }
+}
+}  // namespace
+
+extern "C" int eig_lsqproject_noalloc(eig_ctx *ctx, const int32_t *use, double *acoeffs,
+                                      double *bcoeffs, int32_t *status) {
+  if (!ctx || !use) return eig_set_err(ctx, EIG_E_ARG, "eig_lsqproject_noalloc: args");
+  if (!ctx->have_load)
+    return eig_set_err(ctx, EIG_E_STATE, "eig_lsqproject_noalloc before eig_loadings");
+  if (!ctx->noalloc) return eig_set_err(ctx, EIG_E_STATE, "eig_lsqproject_noalloc: no workspace");
+  EIG_HIP(ctx, hipSetDevice(ctx->device));
+  const int64_t M = ctx->nsnp;
+  const int32_t nind = ctx->nind, K = ctx->last_K;
+  hipStream_t st = ctx->stream;
+  const int nP = K * (K + 1) / 2, W = nP + K + 2;
+  const int64_t nchunk = (M + kLsqChunk - 1) / kLsqChunk;
+  const int32_t nblk = (nind + 63) / 64;
+  // workspace: acc | ans | rhs | A | B | status | use | partials of G chunks
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t per = (size_t)nind * W;
+  const size_t o_acc = 0, o_ans = up(8 * per), o_rhs = o_ans + up(8 * (size_t)K * nind),
+               o_A = o_rhs + up(8 * (size_t)K * nind), o_B = o_A + up(8 * (size_t)K * nind),
+               o_st = o_B + up(8 * (size_t)K * nind), o_use = o_st + up(4 * (size_t)nind),
+               o_part = o_use + up(4 * (size_t)nind);
+  if (o_part + 8 * per > ctx->noalloc_bytes)
+    return eig_set_err(ctx, EIG_E_ARG,
+                       "eig_lsqproject_noalloc: %d individuals x K=%d need more than the %zu-byte "
+                       "static workspace", nind, K, ctx->noalloc_bytes);
+  const int64_t G = std::min<int64_t>(nchunk, (int64_t)((ctx->noalloc_bytes - o_part) / (8 * per)));
+  char *base = (char *)ctx->noalloc;
+  double *d_acc = (double *)(base + o_acc), *d_ans = (double *)(base + o_ans),
+         *d_rhs = (double *)(base + o_rhs), *d_A = (double *)(base + o_A),
+         *d_B = (double *)(base + o_B), *d_part = (double *)(base + o_part);
+  int32_t *d_st = (int32_t *)(base + o_st), *d_use = (int32_t *)(base + o_use);
+  const size_t kn = (size_t)K * nind;
+  EIG_HIP(ctx, hipMemcpyAsync(d_use, use, 4 * (size_t)nind, hipMemcpyHostToDevice, st));
+  // the caller's current outputs go in, so entries the rule leaves alone come back as they were
+  if (acoeffs) EIG_HIP(ctx, hipMemcpyAsync(d_A, acoeffs, 8 * kn, hipMemcpyHostToDevice, st));
+  if (bcoeffs) EIG_HIP(ctx, hipMemcpyAsync(d_B, bcoeffs, 8 * kn, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)((per + 255) / 256)), dim3(256), 0, st, d_acc,
+                     (int64_t)per);
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)((kn + 255) / 256)), dim3(256), 0, st, d_ans,
+                     (int64_t)kn);
+  const double *d_fx = (const double *)ctx->fxscal.p, *d_xm = d_fx + kMaxK;
+  for (int64_t c0 = 0; c0 < nchunk; c0 += G) {
+    const int64_t gc = std::min<int64_t>(G, nchunk - c0);
+    EIG_KSWITCH(K, hipLaunchKernelGGL((k_lsq_gram<KC>), dim3((unsigned)(gc * nblk)), dim3(64), 0,
+                                      st, (const uint8_t *)ctx->geno.p, ctx->rlen, M, nind,
+                                      (const double *)ctx->ffvecs.p, d_fx,
+                                      (const double *)ctx->fancy.p, d_xm,
+                                      (const uint8_t *)ctx->ignore_dev.p, (const double *)nullptr,
+                                      (const int32_t *)nullptr, 0, d_part, nblk, c0));
+    hipLaunchKernelGGL(k_lsq_accum, dim3((unsigned)((per + 255) / 256)), dim3(256), 0, st, d_acc,
+                       (const double *)d_part, gc, (int64_t)per);
+  }
+  hipLaunchKernelGGL(k_lsq_solve, dim3((unsigned)((nind + 127) / 128)), dim3(128), 0, st,
+                     (const double *)d_acc, (int64_t)1, nind, K, (const int32_t *)d_use,
+                     (const int32_t *)nullptr, d_ans, d_rhs, d_st);
+  hipLaunchKernelGGL(k_lsq_stale, dim3(1), dim3(64), 0, st, (const double *)d_ans,
+                     (const double *)d_rhs, (const int32_t *)d_st, nind, K, acoeffs ? d_A : nullptr,
+                     bcoeffs ? d_B : nullptr);
+  EIG_HIP(ctx, hipGetLastError());
+  if (status) EIG_HIP(ctx, hipMemcpyAsync(status, d_st, 4 * (size_t)nind, hipMemcpyDeviceToHost, st));
+  if (acoeffs) EIG_HIP(ctx, hipMemcpyAsync(acoeffs, d_A, 8 * kn, hipMemcpyDeviceToHost, st));
+  if (bcoeffs) EIG_HIP(ctx, hipMemcpyAsync(bcoeffs, d_B, 8 * kn, hipMemcpyDeviceToHost, st));
+  EIG_HIP(ctx, hipStreamSynchronize(st));
+  return EIG_OK;
+}

@@ -179,6 +179,17 @@ int eig_lsqproject(eig_ctx *ctx, const int32_t *use, double *acoeffs, double *bc
 int eig_lsqproject_fill(eig_ctx *ctx, const int32_t *use, const int32_t *itype, int32_t npop,
                         double *acoeffs, double *bcoeffs, int32_t *status);
 
+/* eig_lsqproject that allocates nothing: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid it) once the context is set up.  Same inputs, outputs,
+ * status codes and bits as eig_lsqproject; its partial sums live in a static 64 MiB
+ * device array per GPU (loadings and genotypes stay where the earlier calls put
+ * them).  Returns EIG_E_ARG when nind x (K(K+1)/2 + K + 2) doubles plus the outputs do
+ * not fit it.  Every call on one GPU uses the same array: calls that overlap on a GPU
+ * race, from any contexts or threads. */
+int eig_lsqproject_noalloc(eig_ctx *ctx, const int32_t *use, double *acoeffs, double *bcoeffs,
+                           int32_t *status);
+
 /* fastmode: randomized block-Krylov PCA (kjg_fpca, ksrc/kjg_fpca.c:24-101) of the
  * selected samples over every SNP not marked dead in the last eig_snpstats (the
  * keep rule does not apply, as in the reference), normalised as setgval does
