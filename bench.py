@@ -46,6 +46,14 @@ def one_pass(eng, xindex, form, nvec):
     return ev, vec
 
 
+def host_threads():
+    """The reference's thread rule (all cores up to 8, else one fewer, smartpca.c:1042-1069),
+    capped at the 16 host cores a one-GPU box gives a job."""
+    n = os.cpu_count() or 1
+    n = n if n <= 8 else min(n - 1, 127)
+    return min(n, 16)
+
+
 def cpu_baseline(eng, panel, nthreads):
     """Reference algorithm on the host (the oracle's C restatement): partial-sum
     lookup XTX on a SNP sample + LAPACK dspev at a reduced order, scaled to the
@@ -91,6 +99,29 @@ def cpu_cfg1_full(nthreads, runs=3):
         times.append(time.perf_counter() - t0)
     return dict(s=round(float(np.median(times)), 3), runs=runs, cores=nthreads,
                 N=P.nind, M=P.nsnp, lapack="numpy's bundled scipy_openblas64 dspev")
+
+
+def cpu_full(eng, panel, nthreads):
+    """The reference CPU pass in full at the headline workload: column statistics and
+    partial-sum lookup XTX over every SNP, trace scaling and LAPACK dspev of the N x N
+    matrix, once (minutes of host time)."""
+    from oracle import oracle as O
+    N, M = panel.nind, panel.nsnp
+    xindex = np.arange(N, dtype=np.int32)
+    rows = eng.fetch_packed(0, M)
+    t0 = time.perf_counter()
+    st = O.snpstats(rows, xindex)
+    lower = O.xtx_lookup(rows, xindex, st["levels"], st["status"], nthreads=nthreads)
+    t_x = time.perf_counter() - t0
+    log(f"cpu full pass: statistics + lookup XTX {t_x:.1f} s")
+    t1 = time.perf_counter()
+    full, _ = O.finalize_xtx(lower, N)
+    del lower
+    O.dspev_lower(full)
+    t_e = time.perf_counter() - t1
+    log(f"cpu full pass: trace scaling + dspev {t_e:.1f} s")
+    return dict(s=round(t_x + t_e, 2), xtx_s=round(t_x, 2), eig_s=round(t_e, 2), cores=nthreads,
+                N=N, M=M, lapack="numpy's bundled scipy_openblas64 dspev")
 
 
 def serve_leg(eng, P, vec, seconds, rate=1000.0, pool_n=4096):
@@ -185,9 +216,12 @@ def serve_leg(eng, P, vec, seconds, rate=1000.0, pool_n=4096):
                 chunk=info0["chunk"], grid=info0["grid"])
 
 
-def fastmode_leg(args):
+def fastmode_leg(args, steps=None, warmup=None):
     """fastmode at the 50-population panel's shape (N=100,000 x M=500,000, 1% missing;
-    --scale / --nsnp shrink it): one warm-up and `steps` timed calls per form."""
+    --scale / --nsnp shrink it): `warmup` untimed and `steps` timed calls per form
+    (the run's --steps / --warmup unless given)."""
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     import eig_amd
     from eig_amd import synth
     P = synth.config_panel(4, nsnp=None if args.nsnp == 600000 else args.nsnp, scale=args.scale)
@@ -202,10 +236,10 @@ def fastmode_leg(args):
                       "N": N, "M": M, "K": 10, "L": 20, "I": 10}, "forms": {}}
     ref = None
     for form, name in ((eig_amd.FORM_FP64, "fp64"), (eig_amd.FORM_BF16X4, "bf16x4")):
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             eng.fpca(10, 20, 10, seed=1234, form=form)
         times = []
-        for s in range(args.steps):
+        for s in range(steps):
             t0 = time.perf_counter()
             ev, vec = eng.fpca(10, 20, 10, seed=1234, form=form)
             times.append(time.perf_counter() - t0)
@@ -220,7 +254,8 @@ def fastmode_leg(args):
             rec["evec_maxabs_vs_fp64"] = float(np.max(np.abs(vec - ref[1])))
         out["forms"][name] = rec
     eng.close()
-    print(json.dumps(out))
+    out["speedup_vs_fp64"] = round(out["forms"]["fp64"]["s"] / out["forms"]["bf16x4"]["s"], 3)
+    return out
 
 
 def ridoutlier(evecs, nle, thresh):
@@ -235,7 +270,7 @@ def ridoutlier(evecs, nle, thresh):
     return np.nonzero(bad)[0]
 
 
-def cfg3_leg(args):
+def cfg3_leg(args, max_passes=6):
     """The 25-population admixed panel at N=20,000 x M=1,000,000 (5% admixed, 3% missing)
     through smartpca's outlier loop (5 iterations, 6 sigma, 10 outlier eigenvectors),
     generated in HBM: each form's passes (statistics, XTX, eigensolve) timed, the
@@ -256,7 +291,7 @@ def cfg3_leg(args):
         dead = np.zeros(M, dtype=bool)
         passes, removed = [], []
         t_all = time.perf_counter()
-        for it in range(1, 7):
+        for it in range(1, max_passes + 1):
             xindex = np.nonzero(keep)[0].astype(np.int32)
             t0 = time.perf_counter()
             eng.select(xindex)
@@ -268,7 +303,7 @@ def cfg3_leg(args):
             passes.append(dict(nrows=len(xindex), s=round(dt, 3), syrk_ms=eng.last_ms("syrk"),
                                tridiag_ms=eng.last_ms("tridiag"), trace_sum=float(ev.sum())))
             log(f"cfg3 {name} pass {it}: nrows {len(xindex)} {dt:.3f} s")
-            if it == 6:
+            if it == max_passes:
                 break
             bad = ridoutlier(vec, min(10, len(xindex) - 1), 6.0)
             if len(bad) == 0:
@@ -284,7 +319,7 @@ def cfg3_leg(args):
             np.abs(np.array(b["top10"]) - np.array(f["top10"])) / np.abs(np.array(f["top10"]))))
     out["speedup_vs_fp64"] = round(f["s"] / b["s"], 3)
     eng.close()
-    print(json.dumps(out))
+    return out
 
 
 def main():
@@ -300,6 +335,11 @@ def main():
     ap.add_argument("--forms", default="bf16x4,bf16x3,fp64",
                     help="forms to run; anything but all three prints stage times only "
                          "(for profiler runs)")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="instead of the GPU passes, run the reference CPU pass once in full at "
+                         "the headline workload and print its time")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="leave out the one-call fastmode and one-pass N=20,000 summaries")
     ap.add_argument("--cfg3", action="store_true",
                     help="instead of the XTX pass, run the 25-population admixed panel at "
                          "20,000 x 1,000,000 through the outlier loop in both forms")
@@ -317,10 +357,10 @@ def main():
     import eig_amd
     from eig_amd import synth
     if args.fastmode:
-        fastmode_leg(args)
+        print(json.dumps(fastmode_leg(args)))
         return
     if args.cfg3:
-        cfg3_leg(args)
+        print(json.dumps(cfg3_leg(args)))
         return
     nvec = 10
     P = synth.config_panel(2, nsnp=args.nsnp, scale=args.scale)
@@ -329,6 +369,13 @@ def main():
     eng = eig_amd.Engine(0)
     P.to_device(eng)
     xindex = np.arange(N, dtype=np.int32)
+    if args.cpu_full:
+        out = {"metric": "reference CPU pass s (statistics, lookup XTX, dspev)", "unit": "s",
+               "config": {"workload": "Balding-Nichols 10 pops x 500, 2% missing", "N": N, "M": M},
+               "kind": "port", **cpu_full(eng, P, host_threads())}
+        eng.close()
+        print(json.dumps(out))
+        return
     if args.copy_calib_mb:
         nb = args.copy_calib_mb << 20
         a, b = eng.dev_alloc(nb), eng.dev_alloc(nb)
@@ -444,14 +491,27 @@ def main():
         out["lsqproject_samples_per_s"] = sv["resident"]["samples_per_s"]
         out["serving"] = sv
     if not args.no_cpu:
-        nthreads = os.cpu_count() or 1
-        nthreads = nthreads if nthreads <= 8 else min(nthreads - 1, 127)
-        nthreads = min(nthreads, 16)
+        nthreads = host_threads()
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(eng, P, nthreads)
         log("cpu pass in full at N=1,000 x M=50,000")
         out["cpu_cfg1_full"] = cpu_cfg1_full(nthreads)
     eng.close()
+    if not args.no_extra:
+        # the other headline shapes, one call / one pass each per form (their own
+        # flags run them in full)
+        log("fastmode at N=100,000 x M=500,000, one call per form")
+        fm = fastmode_leg(args, steps=1, warmup=1)
+        out["fastmode"] = {"config": fm["config"], "speedup_vs_fp64": fm["speedup_vs_fp64"],
+                           **{k: {kk: v[kk] for kk in ("s", "power_ms", "finish_ms",
+                                                       "eval_relerr_vs_fp64") if kk in v}
+                              for k, v in fm["forms"].items()}}
+        log("N=20,000 x M=1,000,000, one pass per form")
+        c3 = cfg3_leg(args, max_passes=1)
+        out["cfg3_one_pass"] = {"config": c3["config"], "speedup_vs_fp64": c3["speedup_vs_fp64"],
+                                "top10_eval_relerr_bf16x4_vs_fp64":
+                                    c3.get("top10_eval_relerr_bf16x4_vs_fp64"),
+                                **{k: v["passes"][0] for k, v in c3["forms"].items()}}
     print(json.dumps(out))
 
 

@@ -98,7 +98,7 @@ extern "C" void eig_close(eig_ctx *ctx) {
                     &ctx->levbf,  &ctx->live,    &ctx->xtx,        &ctx->part,     &ctx->ework,
                     &ctx->evecs_dev, &ctx->ffvecs, &ctx->fxscal,   &ctx->projw,    &ctx->proj_out,
                     &ctx->use_dev, &ctx->ignore_dev, &ctx->scratch, &ctx->scratch2,
-                    &ctx->selg,   &ctx->fill,   &ctx->evall};
+                    &ctx->selg,   &ctx->selb,   &ctx->fill,     &ctx->evall};
   for (DevBuf *b : bufs) free_buf(*b);
   if (ctx->hostpin.p) hipHostFree(ctx->hostpin.p);
   hipEventDestroy(ctx->ev0);
@@ -118,7 +118,7 @@ extern "C" double eig_last_ms(const eig_ctx *ctx, const char *what) {
 }
 
 static void invalidate(eig_ctx *ctx) {
-  ctx->have_sel = ctx->have_selg = ctx->have_stats = ctx->have_xtx = ctx->have_load = false;
+  ctx->have_sel = ctx->have_selg = ctx->have_selb = ctx->have_stats = ctx->have_xtx = ctx->have_load = false;
 }
 
 extern "C" int eig_load_packed(eig_ctx *ctx, const uint8_t *rows, int64_t rlen, int64_t nsnp,
@@ -344,7 +344,7 @@ extern "C" int eig_select(eig_ctx *ctx, const int32_t *xindex, int32_t nrows) {
   ctx->nrows = nrows;
   ctx->nst = nst;
   ctx->have_sel = true;
-  ctx->have_selg = ctx->have_stats = ctx->have_xtx = ctx->have_load = false;
+  ctx->have_selg = ctx->have_selb = ctx->have_stats = ctx->have_xtx = ctx->have_load = false;
   return EIG_OK;
 }
 

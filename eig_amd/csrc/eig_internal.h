@@ -45,6 +45,10 @@ struct eig_ctx {
   // first use after each selection)
   DevBuf selg;
   bool have_selg = false;
+  // the split-BF16 SYRK's operand codes: per 64-SNP group, sample and k-slot half,
+  // 16 bytes, each the LDS byte offset of a table entry (built on first use)
+  DevBuf selb;
+  bool have_selb = false;
 
   // per-SNP statistics (device)
   DevBuf c0, c1, nmiss, status, mean, fancy;

@@ -1179,15 +1179,18 @@ static int64_t quiet_ticks(int32_t L, int32_t clock_khz) {
 // store drain) and saves a few us per column on short ones.  With the 64-row
 // w-column chunks, cuts of 500 to 1,000 were best for n = 3,000 and 5,000 (1 to
 // 3,000 tried; never merging costs 1.1 ms at n = 5,000).
+// EIG_NO_MERGE=1 runs every column as k_symv + k_wcol: counter-collection runs use it,
+// since no launch then waits on other blocks of its own grid.
 constexpr int32_t kMergeBelow = 1000;
 static void launch_tridiag(EigWs ws, int32_t n, int32_t clock_khz, hipStream_t st) {
+  const int32_t merge_below = getenv("EIG_NO_MERGE") != nullptr ? 1 : kMergeBelow;
   hipLaunchKernelGGL(k_col0, dim3(1), dim3(256), 0, st, ws, n);
   int32_t base = 0;  // hand-off tickets taken by the earlier columns
   for (int32_t c = 0; c + 1 < n; ++c) {
     const int32_t j = c % kNB;
     const int32_t L = n - c - 1;
     const int32_t gsym = (L + kSymvRows - 1) / kSymvRows, gw = (L + kWRows - 1) / kWRows;
-    if (L >= kMergeBelow) {
+    if (L >= merge_below) {
       const int32_t g4 = (L + kSymvRowsLong - 1) / kSymvRowsLong;
       hipLaunchKernelGGL(k_symv<kSymvRowsLong>, dim3((unsigned)g4), dim3(256), 0, st, ws, n, c, j);
       hipLaunchKernelGGL(k_wcol, dim3((unsigned)gw), dim3(256), 0, st, ws, n, c, j, g4);

@@ -1278,14 +1278,21 @@ extern "C" int eig_fpca_form(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint
   const double *src = U;
   if (r < K) return eig_set_err(ctx, EIG_E_NUMERIC, "fpca: Krylov basis rank %d < K", r);
   // ---- B = X^T U  (npad x r, ld ldq), then the top-K left singular vectors of B
-  const int CTr = (r + 15) / 16;
   if (bf) {
     EIG_TRY(launch_xtb_bf16(ctx, P, (const uint2 *)ctx->selg.p, lbf, tbh, tbl, Mpad, src, ldq, r,
                             part, part_doubles, Bm, (int32_t)ldq, 1.0, bslots, st));
   } else {
-  EIG_TRY(tune_xtb(ctx, F, P, CTr, src, ldq, part, Smax, Bm, (int32_t)ldq));
-  EIG_TRY(launch_xtb(ctx, P, CTr, F.xtb[CTr].grid, F.xtb[CTr].S, src, ldq, r, part, Bm,
-                     (int32_t)ldq, 1.0, st));
+    // in 32-column pieces: the wide instances hold all r columns' accumulators and run
+    // one wave per SIMD, at about half the FP64 rate of k_fpca_xtb<2> (four waves per
+    // SIMD), which the power iteration has tuned already when L <= 32.  A last piece
+    // whose 32 columns would pass ldq (a multiple of 16 >= r) takes 16.
+    for (int32_t c0 = 0; c0 < r; c0 += 32) {
+      const int32_t cw = std::min<int32_t>(32, r - c0);
+      const int ct = c0 + 32 <= ldq ? 2 : 1;
+      EIG_TRY(tune_xtb(ctx, F, P, ct, src + c0, ldq, part, Smax, Bm + c0, (int32_t)ldq));
+      EIG_TRY(launch_xtb(ctx, P, ct, F.xtb[ct].grid, F.xtb[ct].S, src + c0, ldq, cw, part,
+                         Bm + c0, (int32_t)ldq, 1.0, st));
+    }
   }
   EIG_TRY(gemm_nt_long(ctx, r, r, npad, Bm, ldq, Bm, ldq, G, r, part, part_doubles, st));
   // top-K eigenpairs of B^T B (order r up to 256) by the FP64 eigensolver: all

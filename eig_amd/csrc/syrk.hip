@@ -16,6 +16,7 @@
 //                of the top eigenvalues; the fourth product removes it.
 #include <cmath>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "eig_internal.h"
@@ -142,8 +143,9 @@ __global__ __launch_bounds__(256, 2) void k_syrk_f64(const uint32_t *__restrict_
 // Grouped code layout: per sample tile st, 64-SNP group g and sample i, 16 bytes
 // selg[(st * G64 + g) * 128 + i] whose word 2h + (s >> 1), bits 16 (s & 1) + 2 j,
 // hold the code of SNP 64 g + 16 s + 8 h + j (s = 16-SNP stage 0..3, h = half of
-// the stage 0..1, j = 0..7); SNPs past M carry code 3.  A lane of the SYRK that
-// feeds MFMA k-slots 8h..8h+7 of a sample reads one 8-byte word pair per group.
+// the stage 0..1, j = 0..7); SNPs past M carry code 3.  A lane of fastmode's split-
+// BF16 X^T B product that feeds MFMA k-slots 8h..8h+7 of a sample reads one 8-byte
+// word pair per group.
 
 __global__ __launch_bounds__(256) void k_regroup(const uint32_t *__restrict__ sel, int64_t M,
                                                  int64_t G64, int64_t ngroups,
@@ -186,11 +188,57 @@ __global__ __launch_bounds__(256) void k_regroup(const uint32_t *__restrict__ se
 //   read disjoint bank ranges.
 constexpr int kTabWords = 4 * 4 * 2 * 16;
 
+// Byte-code layout for the split-BF16 SYRK: per sample tile st, 64-SNP group g,
+// sample i and k-slot half h, 16 bytes at selb[((st * G64 + g) * 128 + i) * 2 + h];
+// byte 4 s + q is the LDS byte offset, within the (stage s, pair q) row of the level
+// table, of the entry for this sample's code pair: 64 h + 4 c, c = code(2p) |
+// code(2p+1) << 2 for SNPs 2p, 2p+1 of stage s (p = 4h + q).  A lookup is then one
+// byte extract; the stage/pair row and the buffer are immediate offsets.
+__global__ __launch_bounds__(256) void k_regroup8(const uint32_t *__restrict__ sel, int64_t M,
+                                                  int64_t G64, int64_t ngroups,
+                                                  uint4 *__restrict__ selb) {
+  __shared__ uint32_t w[2][kGrp * kTileWords];
+  const int half = threadIdx.x >> 7, i = threadIdx.x & 127;
+  const int64_t gid = (int64_t)blockIdx.x * 2 + half;  // over nst * G64 groups
+  const bool live = gid < ngroups;
+  const int64_t st = live ? gid / G64 : 0, g = live ? gid % G64 : 0;
+  const uint32_t *src = sel + (st * M + g * kGrp) * kTileWords;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = i + 128 * q;  // SNP e >> 3 of the group, word e & 7
+    w[half][e] = (live && g * kGrp + (e >> 3) < M) ? src[e] : 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  if (!live) return;
+  const int wd = i >> 4, sh = 2 * (i & 15);
+  auto code = [&](int m) { return (w[half][m * kTileWords + wd] >> sh) & 3u; };
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    uint32_t o[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      uint32_t word = 0u;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int m = 16 * s + 2 * (4 * h + q);
+        const uint32_t c = code(m) | (code(m + 1) << 2);
+        word |= (64u * (uint32_t)h + 4u * c) << (8 * q);
+      }
+      o[s] = word;
+    }
+    selb[(gid * kTileS + i) * 2 + h] = make_uint4(o[0], o[1], o[2], o[3]);
+  }
+}
+
+// The SYRK over the byte-code layout: per lookup one byte extract and two LDS loads
+// whose table (hi, lo), stage row and buffer are immediate offsets (the group loop is
+// unrolled by two so the buffer is a constant); 2 % faster than 4-bit codes with the
+// lane's table offset added per lookup (measured at N=5,000 x M=600,000).
 // kGrpBlk: the FP32 block is a whole number of groups, so promotions happen only at
 // group ends and the four stages of a group form one branch-free block of code.
 template <bool kLoLo, bool kGrpBlk>
 __global__ __launch_bounds__(256, 2) void k_syrk_bf16(
-    const uint2 *__restrict__ selg, int64_t G64, int64_t M, const uint32_t *__restrict__ levbf,
+    const uint4 *__restrict__ selb, int64_t G64, int64_t M, const uint32_t *__restrict__ levbf,
     int32_t P, int64_t chunk, int32_t blk, double *__restrict__ part) {
   // [buf][hi/lo][entry]; the lo table sits 2052 bytes after the hi table, an offset
   // no paired LDS load can span, so each lookup lands in its operand register
@@ -206,21 +254,15 @@ __global__ __launch_bounds__(256, 2) void k_syrk_bf16(
   const int lane = t & 63, w = t >> 6;
   const int wr = w >> 1, wc = w & 1;
   const int h = lane >> 5;
-
-  // codes: rows A0, A1 (tile ti) and B0, B1 (tile tj), half h, as uint2 per group
-  // (row r of the four at src + roff[r]; B's offset from A is wave-uniform)
   const int64_t g0 = m_begin / kGrp;
-  const uint2 *src = selg + (((int64_t)ti * G64 + g0) * kTileS + 64 * wr + (lane & 31)) * 2 + h;
+  const uint4 *src = selb + (((int64_t)ti * G64 + g0) * kTileS + 64 * wr + (lane & 31)) * 2 + h;
   const int64_t offB = ((int64_t)(tj - ti) * G64 * kTileS + 64 * (wc - wr)) * 2;
   const int64_t roff[4] = {0, 64, offB, offB + 64};
-  // table entries built by this thread: e = t and t + 256
-  //   s = e >> 7, q = (e >> 5) & 3, hh = (e >> 4) & 1, c = e & 15; pair p = 4 hh + q
   auto lev_fetch = [&](int32_t gi, uint32_t (&lv)[4]) {
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       const int e = t + 256 * u;
       const int s = e >> 7, q = (e >> 5) & 3, hh = (e >> 4) & 1, c = e & 15;
-      // SNPs past m_end read SNP m_begin (in bounds) and take 0: no branch
       const int64_t m0 = m_begin + (int64_t)gi * kGrp + 16 * s + 2 * (4 * hh + q);
       const uint32_t e0 = levbf[(m0 < m_end ? m0 : m_begin) * 4 + (c & 3)];
       const uint32_t e1 = levbf[(m0 + 1 < m_end ? m0 + 1 : m_begin) * 4 + (c >> 2)];
@@ -237,7 +279,6 @@ __global__ __launch_bounds__(256, 2) void k_syrk_bf16(
       tab[buf][1][e] = (e0 >> 16) | (e1 & 0xFFFF0000u);
     }
   };
-
   f32x16 acc[2][2];
   double dacc[2][2][16];
 #pragma unroll
@@ -261,11 +302,10 @@ __global__ __launch_bounds__(256, 2) void k_syrk_bf16(
           acc[a][b][r] = 0.f;
         }
   };
-  const int32_t kper = kGrpBlk ? blk / kGrp : blk / 16;  // groups / stages per FP32 block
-  int32_t left = kper;  // groups / stages until the next promotion
-
+  const int32_t kper = kGrpBlk ? blk / kGrp : blk / 16;
+  int32_t left = kper;
   uint32_t lv[4];
-  uint2 cur[4], nxt[4];
+  uint4 cur[4], nxt[4];
   if (ng > 0) {
     lev_fetch(0, lv);
 #pragma unroll
@@ -273,33 +313,27 @@ __global__ __launch_bounds__(256, 2) void k_syrk_bf16(
     tab_store(0, lv);
   }
   __syncthreads();
-  const uint32_t hoff = 16u * (uint32_t)h;
-  for (int32_t gi = 0; gi < ng; ++gi) {
-    const int buf = gi & 1;
-    // the next group (the last group loads itself again, unused)
+  const char *tbase = (const char *)&tab[0][0][0];
+  constexpr int kTabBytes = (kTabWords + 1) * 4;  // one table; hi then lo, per buffer
+  auto group = [&](auto bufc, int32_t gi) {
+    constexpr int buf = decltype(bufc)::value;
     const int32_t gn = gi + 1 < ng ? gi + 1 : gi;
-    // this lane's half of the tables: the lane term joins the base once per group, so
-    // a lookup is one extract and one shift-add, the rest an immediate offset
-    const uint32_t *th = tab[buf][0] + hoff, *tl = tab[buf][1] + hoff;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      // the next group's codes and level entries go out once stage 1 has used the
-      // low words (their registers are free then)
       if (s == 2) {
         lev_fetch(gn, lv);
 #pragma unroll
         for (int r = 0; r < 4; ++r) nxt[r] = src[roff[r] + (int64_t)gn * (2 * kTileS)];
       }
-      // operand dwords of row r: q-th pair of this lane's half of stage s
       auto row = [&](int r, bf16x8 &hi, bf16x8 &lo) {
-        const uint32_t word = (s < 2) ? cur[r].x : cur[r].y;
+        const uint32_t word = s == 0 ? cur[r].x : s == 1 ? cur[r].y : s == 2 ? cur[r].z : cur[r].w;
         uint32_t hv[4], lw[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const uint32_t c = (word >> (16 * (s & 1) + 4 * q)) & 15u;
-          const uint32_t idx = (uint32_t)(((s * 4 + q) * 2) * 16) + c;
-          hv[q] = th[idx];
-          lw[q] = tl[idx];
+          const uint32_t off = (word >> (8 * q)) & 0xFFu;
+          const char *pe = tbase + off + (2 * buf * kTabBytes + (s * 4 + q) * 128);
+          hv[q] = *(const uint32_t *)pe;
+          lw[q] = *(const uint32_t *)(pe + kTabBytes);
         }
         typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t u32x4;
         u32x4 hh = {hv[0], hv[1], hv[2], hv[3]}, ll = {lw[0], lw[1], lw[2], lw[3]};
@@ -322,7 +356,6 @@ __global__ __launch_bounds__(256, 2) void k_syrk_bf16(
           acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[b], acc[a][b], 0, 0, 0);
         }
       }
-      // promote the FP32 block sums into FP64 every blk SNPs
       if (!kGrpBlk && --left == 0) {
         promote();
         left = kper;
@@ -336,8 +369,12 @@ __global__ __launch_bounds__(256, 2) void k_syrk_bf16(
 #pragma unroll
     for (int r = 0; r < 4; ++r) cur[r] = nxt[r];
     __syncthreads();
+  };
+  for (int32_t gi = 0; gi < ng; gi += 2) {
+    group(std::integral_constant<int, 0>{}, gi);
+    if (gi + 1 < ng) group(std::integral_constant<int, 1>{}, gi + 1);
   }
-  promote();  // the last, partial block (adds zeros when already promoted)
+  promote();
   double *out = part + ((int64_t)split * P + pair) * (kTileS * kTileS);
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -429,6 +466,21 @@ int eig_ensure_selg(eig_ctx *ctx) {
   return EIG_OK;
 }
 
+static int eig_ensure_selb(eig_ctx *ctx) {
+  if (ctx->have_selb) return EIG_OK;
+  const int64_t M = ctx->nsnp;
+  const int64_t G64 = (M + kGrp - 1) / kGrp;
+  const int64_t ngroups = (int64_t)ctx->nst * G64;
+  EIG_TRY(eig_ensure(ctx, ctx->selb, (size_t)ngroups * kTileS * 32));
+  eig_time_begin(ctx);
+  hipLaunchKernelGGL(k_regroup8, dim3((unsigned)((ngroups + 1) / 2)), dim3(256), 0, ctx->stream,
+                     (const uint32_t *)ctx->sel.p, M, G64, ngroups, (uint4 *)ctx->selb.p);
+  EIG_HIP(ctx, hipGetLastError());
+  eig_time_end(ctx, "regroup");
+  ctx->have_selb = true;
+  return EIG_OK;
+}
+
 int eig_run_syrk(eig_ctx *ctx, int form, int bf16_block, double *trace_scale) {
   const int64_t M = ctx->nsnp;
   const int32_t T = ctx->nst;
@@ -446,7 +498,7 @@ int eig_run_syrk(eig_ctx *ctx, int form, int bf16_block, double *trace_scale) {
   // multiples of the FP32 block (bf16) or of 256 SNPs and hold at least 2048 SNPs.
   const int64_t align = split ? std::lcm<int64_t>(bf16_block, kGrp) : 256;
   const bool grpblk = split && bf16_block % kGrp == 0;
-  typedef void (*bf16_kernel)(const uint2 *, int64_t, int64_t, const uint32_t *, int32_t, int64_t,
+  typedef void (*bf16_kernel)(const uint4 *, int64_t, int64_t, const uint32_t *, int32_t, int64_t,
                               int32_t, double *);
   bf16_kernel kb = nullptr;
   if (form == EIG_FORM_BF16X3)
@@ -482,14 +534,14 @@ int eig_run_syrk(eig_ctx *ctx, int form, int bf16_block, double *trace_scale) {
   double *d_tr = (double *)ctx->part.p + tile * P * nsplit;
   unsigned grid = (unsigned)(P * nsplit);
   const int64_t G64 = (M + kGrp - 1) / kGrp;
-  if (split) EIG_TRY(eig_ensure_selg(ctx));
+  if (split) EIG_TRY(eig_ensure_selb(ctx));
   eig_time_begin(ctx);
   if (form == EIG_FORM_FP64) {
     hipLaunchKernelGGL(k_syrk_f64, dim3(grid), dim3(256), 0, ctx->stream,
                        (const uint32_t *)ctx->sel.p, M, (const double *)ctx->lev.p, P, chunk,
                        (double *)ctx->part.p);
   } else {
-    hipLaunchKernelGGL(kb, dim3(grid), dim3(256), 0, ctx->stream, (const uint2 *)ctx->selg.p,
+    hipLaunchKernelGGL(kb, dim3(grid), dim3(256), 0, ctx->stream, (const uint4 *)ctx->selb.p,
                        G64, M, (const uint32_t *)ctx->levbf.p, P, chunk, bf16_block,
                        (double *)ctx->part.p);
   }
