@@ -319,7 +319,10 @@ __device__ inline void symv_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
 // Panel vectors are stored row-major, V[r * kNB + q], so a thread loads its row's
 // coefficients in one batch; entries of columns not yet in this panel are stale
 // but always meet a zero coefficient.
-template <class Wait>
+// kEarly (a launch after the symv part's): y and the y.v partials are final before
+// the launch starts, so their loads go out with the others and the part runs on one
+// round of memory latency instead of two.
+template <bool kEarly, class Wait>
 __device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t j, int32_t blk,
                                  int32_t nps, Wait &&wait) {
   __shared__ double red[4][4];
@@ -367,6 +370,12 @@ __device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
     const int q = threadIdx.x & (kNB - 1);
     atc = (threadIdx.x < kNB) ? ws.V[(int64_t)c1 * kNB + q] : ws.W[(int64_t)c1 * kNB + q];
   }
+  double ye = 0.0, yc1e = 0.0, ype[8];
+  if constexpr (kEarly) {
+    ye = ws.y[rs];
+    yc1e = ws.y[c1];
+    load8(ws.ps, 1, threadIdx.x, 256, nps, ype);
+  }
   double praw = 0.0;
   {
     const int comp = lane;  // 0..31 V dots, 32..63 W dots
@@ -387,9 +396,10 @@ __device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
   }
   if (!wait()) return;
   // ---- the symv part's outputs: y and the per-block partials of y.v
-  double yy = ws.y[rs];
-  const double yc1 = ws.y[c1];
-  double yv = sum_strided(ws.ps, 1, threadIdx.x, 256, nps);
+  double yy = kEarly ? ye : ws.y[rs];
+  const double yc1 = kEarly ? yc1e : ws.y[c1];
+  double yv = kEarly ? sum_rest(ws.ps, 1, threadIdx.x, 256, nps, ype)
+                     : sum_strided(ws.ps, 1, threadIdx.x, 256, nps);
   for (int o = 32; o > 0; o >>= 1) yv += __shfl_down(yv, o, 64);
   if (lane == 0) red[wv][0] = yv;
   pqr[wv][lane] = praw;
@@ -508,7 +518,7 @@ __global__ __launch_bounds__(256) void k_symv(EigWs ws, int32_t n, int32_t c, in
   symv_part<R>(ws, n, c, j, (int32_t)blockIdx.x);
 }
 __global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, int32_t j, int32_t nps) {
-  wcol_part(ws, n, c, j, (int32_t)blockIdx.x, nps, []() { return true; });
+  wcol_part<true>(ws, n, c, j, (int32_t)blockIdx.x, nps, []() { return true; });
 }
 
 // ---------------------------------------------------------------- k_column
@@ -525,6 +535,10 @@ __global__ __launch_bounds__(256) void k_wcol(EigWs ws, int32_t n, int32_t c, in
 // first sleep for quiet_ticks of the device's wall clock (most of the symv part's
 // expected duration), then poll.  The counter is cleared by k_col0 and grows
 // monotonically over the columns: column c's tickets are base .. base + gsym - 1.
+// Ordering rests on the ISA, not on a release: the y and partial stores write through
+// the XCD's L2 (sc1) and are drained (vmcnt 0) before the ticket is added, and the
+// waiter polls with sc1 loads and runs an agent-scope acquire (L2 invalidate) before
+// reading them.  A release would write back the whole L2 once per block.
 constexpr int64_t kHandoffSpin = 1 << 22;
 __global__ __launch_bounds__(256) void k_column(EigWs ws, int32_t n, int32_t c, int32_t j,
                                                 int32_t base, int64_t quiet_ticks) {
@@ -563,7 +577,7 @@ __global__ __launch_bounds__(256) void k_column(EigWs ws, int32_t n, int32_t c, 
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     return true;
   };
-  wcol_part(ws, n, c, j, (int32_t)blockIdx.x, gsym, wait);
+  wcol_part<false>(ws, n, c, j, (int32_t)blockIdx.x, gsym, wait);
 }
 
 // A[r][s] -= sum_q V[r][q] W[s][q] + W[r][q] V[s][q] for r, s >= k0: one FP64 MFMA
@@ -700,7 +714,7 @@ __global__ void k_gersh(EigWs ws, int32_t n) {
 // Two Sturm counts (eigenvalues of T below x and below x2), interleaved.  The
 // reciprocal is v_rcp_f64 refined by two Newton steps (relative error ~1e-16),
 // which perturbs each pivot far below the bisection tolerance.  d and e2 are read
-// four entries ahead of use.
+// a batch of four entries ahead of use.
 // number of eigenvalues of T below x (Sturm count of T - x I), with q clamped away
 // from zero by pivmin as in LAPACK's dstebz
 __device__ inline int32_t sturm1(const double *__restrict__ d, const double *__restrict__ e2,
@@ -715,14 +729,44 @@ __device__ inline int32_t sturm1(const double *__restrict__ d, const double *__r
     q = (di - x) - ei * r;
     if (q <= pivmin) { ++cnt; q = fmin(q, -pivmin); }
   };
+  // batches of four steps in two alternating register sets: the next batch's
+  // (wave-uniform, scalar) loads are issued before the current batch's steps, so their
+  // latency hides behind the recurrence
   int32_t i = 1;
-  for (; i + 3 < n; i += 4) {
-    const double d0 = d[i], d1 = d[i + 1], d2 = d[i + 2], d3 = d[i + 3];
-    const double f0 = e2[i - 1], f1 = e2[i], f2 = e2[i + 1], f3 = e2[i + 2];
-    step(d0, f0);
-    step(d1, f1);
-    step(d2, f2);
-    step(d3, f3);
+  double ad[4], af[4], bd[4], bf[4];
+  auto ld = [&](int32_t j0, double (&xd)[4], double (&xf)[4]) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      xd[u] = d[j0 + u];
+      xf[u] = e2[j0 - 1 + u];
+    }
+  };
+  auto run = [&](const double (&xd)[4], const double (&xf)[4]) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) step(xd[u], xf[u]);
+  };
+  if (i + 3 < n) {
+    ld(i, ad, af);
+    for (;;) {
+      // set a holds steps i .. i+3 (all < n)
+      if (i + 7 >= n) {
+        run(ad, af);
+        i += 4;
+        break;
+      }
+      ld(i + 4, bd, bf);
+      run(ad, af);
+      i += 4;
+      // set b holds steps i .. i+3
+      if (i + 7 >= n) {
+        run(bd, bf);
+        i += 4;
+        break;
+      }
+      ld(i + 4, ad, af);
+      run(bd, bf);
+      i += 4;
+    }
   }
   for (; i < n; ++i) step(d[i], e2[i - 1]);
   return cnt;
