@@ -129,7 +129,8 @@ def serve_leg(eng, P, vec, seconds, rate=1000.0, pool_n=4096):
     samples/s, request sizes log-uniform on [1, 512], query samples from the panel's
     populations with per-sample missingness ~ U(0, 0.9), rows resident in HBM.
     Returns latency percentiles (submit -> result ready), samples/s, and the same
-    for a host-memory (PCIe-inclusive) run and a saturating burst."""
+    for a host-memory (PCIe-inclusive) run, a run at 50x the rate (batches shared by
+    requests that arrive while others are on the device) and a saturating burst."""
     import threading
 
     import eig_amd
@@ -151,11 +152,11 @@ def serve_leg(eng, P, vec, seconds, rate=1000.0, pool_n=4096):
     log(f"server up: chunk {info0['chunk']} grid {info0['grid']}")
     rng = np.random.default_rng(0xE16 + 5)
 
-    def schedule(dur):
+    def schedule(dur, r=rate):
         t, out = 0.0, []
         while True:
             n = int(min(512, max(1, np.floor(np.exp(rng.uniform(0, np.log(513)))))))
-            t += rng.exponential(1.0 / (rate / 81.9))
+            t += rng.exponential(1.0 / (r / 81.9))
             if t > dur:
                 return out
             out.append((t, n, int(rng.integers(0, pool_n - n))))
@@ -205,13 +206,20 @@ def serve_leg(eng, P, vec, seconds, rate=1000.0, pool_n=4096):
     log(f"serving (HBM-resident rows): {res}")
     pcie = run(schedule(seconds / 2), False)
     log(f"serving (host rows, PCIe-inclusive): {pcie}")
+    # under load: 50x the headline rate (about 30 % of saturation), where requests
+    # arrive while others are on the device and share batches
+    b0 = srv.info()["batches"]
+    loaded = run(schedule(seconds / 4, 50.0 * rate), True)
+    loaded["batches"] = srv.info()["batches"] - b0
+    loaded["requests_per_batch"] = round(loaded["requests"] / max(loaded["batches"], 1), 2)
+    log(f"serving under load ({50.0 * rate:g} samples/s offered): {loaded}")
     burst = [(0.0, n, off) for (_, n, off) in schedule(seconds / 2)]
     sat = run(burst, True)
     log(f"serving saturation burst: {sat}")
     srv.stop()
     eng.host_free(host_ptr)
     eng.dev_free(pool)
-    return dict(resident=res, host_rows=pcie, burst=dict(samples=sat["samples"],
+    return dict(resident=res, host_rows=pcie, loaded=loaded, burst=dict(samples=sat["samples"],
                 samples_per_s=sat["samples_per_s"], p99_ms=sat["p99_ms"]),
                 chunk=info0["chunk"], grid=info0["grid"])
 
