@@ -276,12 +276,21 @@ __global__ __launch_bounds__(256) void k_select(const uint8_t *__restrict__ geno
                                                 int64_t nsnp, const int32_t *__restrict__ xindex,
                                                 int32_t nrows, int32_t nst, int32_t nsb,
                                                 int32_t ident, uint32_t *__restrict__ sel) {
-  extern __shared__ uint8_t rows[];  // nsb (<= kSelSnps) x rlen
+  extern __shared__ __align__(16) uint8_t rows[];  // nsb (<= kSelSnps) x rlen
   const int64_t m0 = (int64_t)blockIdx.x * nsb;
   const int ns = (int)min((int64_t)nsb, nsnp - m0);
   const uint8_t *src = geno + m0 * rlen;
   const int64_t nbytes = (int64_t)ns * rlen;
-  for (int64_t b = threadIdx.x; b < nbytes; b += blockDim.x) rows[b] = src[b];
+  // 16-byte copies when the block's rows start 16-byte aligned (rlen * nsb a multiple
+  // of 16, as for the 1,250-byte rows of 5,000 samples), bytes for the rest
+  int64_t b16 = 0;
+  if (((uintptr_t)src & 15) == 0) {
+    b16 = nbytes & ~(int64_t)15;
+    const uint4 *s4 = (const uint4 *)src;
+    uint4 *r4 = (uint4 *)rows;
+    for (int64_t b = threadIdx.x; b < b16 / 16; b += blockDim.x) r4[b] = s4[b];
+  }
+  for (int64_t b = b16 + threadIdx.x; b < nbytes; b += blockDim.x) rows[b] = src[b];
   __syncthreads();
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int s = lane >> 3, ww = lane & 7;

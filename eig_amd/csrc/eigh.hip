@@ -1124,25 +1124,31 @@ __global__ __launch_bounds__(256) void k_bt_step(EigWs ws, int32_t n, int32_t p,
   }
 }
 
-// Z_p = T_p (V_p^T X): the partial dots of k_bt_step's nblk blocks summed in block
-// order (three interleaved strands per component, combined in strand order).
+// Z_p = T_p (V_p^T X) from the partial dots of k_bt_step's nblk blocks: one block per
+// vector o.  Thread (g, k) sums the partials
+// b = g, g + 32, ... of entry (k, o), all its loads in one or two rounds; the 32 group
+// sums are then added in group order and multiplied by T_p.
 __global__ __launch_bounds__(1024) void k_bt_reduce(EigWs ws, int32_t p, int32_t last,
                                                     int32_t nvec, int32_t nblk) {
-  __shared__ double acc[3][kNB * kMaxVec];
+  __shared__ double acc[32][kNB + 1];
+  __shared__ double col[kNB];
+  const int o = blockIdx.x;
   const int nd = kNB * nvec;
   const int32_t nb = min(kNB, last - p * kNB + 1);
-  for (int t = threadIdx.x; t < 3 * nd; t += 1024) {
-    const int g = t / nd, idx = t % nd;
-    acc[g][idx] = sum_strided(ws.part1 + idx, nd, g, 3, nblk);
+  const int k = threadIdx.x % kNB, g = threadIdx.x / kNB;
+  acc[g][k] = sum_strided(ws.part1 + k * nvec + o, nd, g, 32, nblk);
+  __syncthreads();
+  if (threadIdx.x < kNB) {
+    double s = acc[0][k];
+    for (int gg = 1; gg < 32; ++gg) s += acc[gg][k];
+    col[k] = s;
   }
   __syncthreads();
-  const double *T = ws.Tall + (int64_t)p * kNB * kNB;
-  for (int idx = threadIdx.x; idx < nd; idx += 1024) acc[0][idx] += acc[1][idx] + acc[2][idx];
-  __syncthreads();
-  for (int idx = threadIdx.x; idx < nd; idx += 1024) {
-    const int q = idx / nvec, o = idx % nvec;
+  if (threadIdx.x < kNB) {
+    const int q = threadIdx.x;
+    const double *T = ws.Tall + (int64_t)p * kNB * kNB;
     double s = 0.0;
-    for (int k = q; k < nb; ++k) s += T[q * kNB + k] * acc[0][k * nvec + o];
+    for (int kk = q; kk < nb; ++kk) s += T[q * kNB + kk] * col[kk];
     ws.Z[q * kMaxVec + o] = s;
   }
 }
@@ -1374,7 +1380,8 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
         const unsigned g = (unsigned)((n - c0 - 1 + kBtRows - 1) / kBtRows);
         hipLaunchKernelGGL(k_bt_step, dim3(g), dim3(256), 0, st, ws, n, p, last, nb,
                            (int32_t)(p < p_last));
-        hipLaunchKernelGGL(k_bt_reduce, dim3(1), dim3(1024), 0, st, ws, p, last, nb, (int32_t)g);
+        hipLaunchKernelGGL(k_bt_reduce, dim3((unsigned)nb), dim3(1024), 0, st, ws, p, last, nb,
+                           (int32_t)g);
       }
       hipLaunchKernelGGL(k_bt_apply0, dim3((unsigned)((n - 1 + 255) / 256)), dim3(256), 0, st, ws,
                          n, last, nb);
