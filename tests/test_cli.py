@@ -97,6 +97,23 @@ def test_reference_default_of_an_unimplemented_parameter_is_accepted(golden, tmp
     assert r.returncode != 0 and "length mismatch" in r.stderr
 
 
+def test_parfile_macros_are_substituted_longest_first(tmp_path):
+    """getpars.c:378-449: upper-case keys are macros replaced in the other values,
+    longest name first (so D2 is not read as D followed by "2")."""
+    (tmp_path / "par").write_text(
+        "D: /nonexistent-d\n"
+        f"D2: {tmp_path}/sub\n"
+        "genotypename: D2/x.geno\n"
+        "snpname: D2/x.snp\n"
+        "indivname: D/x.ind\n")
+    r = run_cli(str(tmp_path / "par"), tmp_path, expect_ok=False)
+    out = r.stdout + r.stderr
+    assert r.returncode != 0
+    assert f"genotypename: {tmp_path}/sub/x.geno" in out
+    assert "indivname: /nonexistent-d/x.ind" in out
+    assert f"can't open {tmp_path}/sub/x.snp" in out
+
+
 def test_geno_hash_mismatch_is_fatal(golden, tmp_path):
     _copy_example(golden, tmp_path)
     lines = (tmp_path / "packed.ind").read_text().splitlines()
@@ -203,6 +220,34 @@ def test_cli_matches_oracle_on_synthetic_panel(tmp_path, form):
     assert [r[2] for r in grows] == [r[2] for r in wrows]
     diff = np.max(np.abs(np.array([r[1] for r in grows]) - np.array([r[1] for r in wrows])))
     assert diff <= 2.01e-4, diff  # %10.4f printing: at most a last-digit flip
+
+
+@pytest.mark.gpu
+def test_cli_poplist_axis_and_projection_match_oracle(tmp_path):
+    """poplistname (egsubs.c:86-109, smartpca.c:774-892): the PCA axes come from the
+    listed populations only; every other sample is projected by lsqproject.  .eval and
+    .evec (axis and projected rows) against the oracle, to print precision."""
+    P, prefix = _synthetic(tmp_path, 4000, 0.2)
+    (tmp_path / "pops.txt").write_text("P01\nP03\n")
+    par = dict(genotypename=prefix + ".geno", snpname=prefix + ".snp",
+               indivname=prefix + ".ind", evecoutname="out.evec", evaloutname="out.eval",
+               numoutevec=4, numoutlieriter=0, poplistname=str(tmp_path / "pops.txt"))
+    write_par(tmp_path / "par", **par)
+    print(f"[gpu-case] cli poplist N={P.nind} M={P.nsnp}", flush=True)
+    run_cli(str(tmp_path / "par"), tmp_path)
+    ref = O.smartpca({k: str(v) for k, v in par.items()}, str(tmp_path))
+    assert len(ref["xindex"]) < P.nind  # the axes use a subset
+    got_eval = np.array([float(x) for x in (tmp_path / "out.eval").read_text().split()])
+    want_eval = np.array([float(x) for x in ref["eval_lines"]])
+    assert len(got_eval) == len(want_eval)
+    assert np.max(np.abs(got_eval - want_eval)) <= 1.01e-6
+    gh, grows = _parse_evec((tmp_path / "out.evec").read_text())
+    wh, wrows = _parse_evec("\n".join(ref["evec_lines"]))
+    assert np.allclose(gh, wh, atol=1.01e-3)
+    assert [r[0] for r in grows] == [r[0] for r in wrows]
+    assert len(grows) == P.nind  # projected samples are written too
+    diff = np.max(np.abs(np.array([r[1] for r in grows]) - np.array([r[1] for r in wrows])))
+    assert diff <= 2.01e-4, diff
 
 
 @pytest.mark.gpu
