@@ -8,9 +8,13 @@ and the FP64 eigensolve (all eigenvalues + top-10 eigenvectors).  The chosen for
 beside it.  Genotypes are generated directly in HBM (seeded Balding-Nichols,
 10 populations x 500, Fst ~ U(0.01,0.1), p ~ U(0.01,0.5), 2% missing) before the
 timed region.  The lsqproject serving leg reports p50/p99 latency at 1,000
-samples/s Poisson arrivals against the same panel.
+samples/s Poisson arrivals against the same panel (and at 50x that rate, and a
+saturating burst).  The same line carries the reference CPU pass (a sampled estimate
+at this shape and a full run at N=1,000 x M=50,000), one fastmode call per form at
+N=100,000 x M=500,000 and one pass per form at N=20,000 x M=1,000,000.
 
-Prints one JSON line.
+Other modes: --fastmode and --cfg3 run those shapes in full; --cpu-full times the
+reference CPU pass in full at the headline shape.  Prints one JSON line.
 """
 from __future__ import annotations
 
