@@ -586,91 +586,121 @@ __global__ __launch_bounds__(256) void k_column(EigWs ws, int32_t n, int32_t c, 
 typedef __attribute__((__vector_size__(4 * sizeof(double)))) double eig_f64x4;
 constexpr int kS2K = 2 * kNB;        // GEMM depth
 constexpr int kS2Ld = kS2K + 1;      // padded LDS row (conflict-free 16-row reads)
-// One block per 64-row x kS2Strip-tile strip: the strip's [V|W] rows are staged
-// once and reused for every 64x64 column tile, whose [W|V] rows are staged in turn.
-constexpr int kS2Strip = 2;
+
+// 64 consecutive rows of two row-major n x kNB panels (P0 and P1), 16 bytes per load:
+// element i of a thread covers row (t + 256 i) / 16, columns 2 ((t + 256 i) % 16) + {0, 1}.
+// Rows at or past n load row n - 1 and are zeroed when stored.
+struct S2Rows {
+  double2 p[2][4];
+};
+__device__ __forceinline__ void s2_load(const double *P0, const double *P1, int32_t r0,
+                                        int32_t n, S2Rows &s) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int e = threadIdx.x + 256 * i;
+    const int64_t off = (int64_t)min(r0 + (e >> 4), n - 1) * kNB + 2 * (e & 15);
+    s.p[0][i] = *(const double2 *)(P0 + off);
+    s.p[1][i] = *(const double2 *)(P1 + off);
+  }
+}
+// row o of `sm`: P0's kNB columns, then P1's; columns at or past nb are zero
+__device__ __forceinline__ void s2_store(double (*sm)[kS2Ld], int32_t r0, int32_t n,
+                                         int32_t nb, const S2Rows &s) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int e = threadIdx.x + 256 * i, o = e >> 4, q = 2 * (e & 15);
+    const bool ok = r0 + o < n;
+    const bool ok0 = ok && q < nb, ok1 = ok && q + 1 < nb;
+    sm[o][q] = ok0 ? s.p[0][i].x : 0.0;
+    sm[o][q + 1] = ok1 ? s.p[0][i].y : 0.0;
+    sm[o][kNB + q] = ok0 ? s.p[1][i].x : 0.0;
+    sm[o][kNB + q + 1] = ok1 ? s.p[1][i].y : 0.0;
+  }
+}
+
+// One block per 64x64 tile.  The B operand rows [W_c | V_c] are staged in LDS; each
+// lane's A operand stays in registers: the MFMA sums over k, so lane group g = lane / 16
+// takes k = 16 g + ks at step ks (in A and B alike), and its A entries are 16
+// consecutive doubles of a row of V (g < 2) or W.  33 KB of LDS and ~150 registers
+// give three blocks per CU.
 __global__ __launch_bounds__(256) void k_syr2k(EigWs ws, int32_t n, int32_t k0, int32_t nb) {
-  __shared__ double sa[64][kS2Ld], sb[64][kS2Ld];
+  __shared__ double sb[64][kS2Ld];
   const int32_t nt = (n - k0 + 63) / 64;
-  const int32_t ns = (nt + kS2Strip - 1) / kS2Strip;
-  const int32_t tr = blockIdx.x / ns, ts = blockIdx.x % ns;
-  const int32_t rb = k0 + tr * 64;
-  // stage: row o of the tile, depth index k (V for k < kNB, W above) -- contiguous rows
-  for (int idx = threadIdx.x; idx < 64 * kS2K; idx += 256) {
-    const int o = idx / kS2K, k = idx % kS2K, q = k & (kNB - 1);
-    const int32_t r = rb + o;
-    const double *src_a = (k < kNB) ? ws.V : ws.W;  // A row r: [V_r | W_r]
-    sa[o][k] = (r < n && q < nb) ? src_a[(int64_t)r * kNB + q] : 0.0;
-  }
+  const int32_t tr = blockIdx.x / nt, tc = blockIdx.x % nt;
+  const int32_t rb = k0 + tr * 64, cb = k0 + tc * 64;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int wr = w >> 1, wc = w & 1;
+  const int wr = w >> 1, wc = w & 1, g = lane >> 4;
   const int32_t lda = ws.lda;
-  // the output tile's old values (row, column of lane's accumulator elements)
-  auto load_old = [&](int32_t cb, double (&old)[2][2][4]) {
+  // the B rows and the A entries are loaded together (both from L2-resident panels)
+  S2Rows rbv;
+  s2_load(ws.W, ws.V, cb, n, rbv);
+  double2 av2[2][8];
+  {
+    const double *asrc = (g < 2 ? ws.V : ws.W) + 16 * (g & 1);
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < 2; ++a) {
+      // rows past n read row n - 1 (their results are not stored)
+      const int32_t r = min(rb + 32 * wr + 16 * a + (lane & 15), n - 1);
+      const double2 *src = (const double2 *)(asrc + (int64_t)r * kNB);
 #pragma unroll
-      for (int b = 0; b < 2; ++b)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int32_t r = rb + 32 * wr + 16 * a + (lane >> 4) + 4 * e;
-          const int32_t c = cb + 32 * wc + 16 * b + (lane & 15);
-          old[a][b][e] = (r < n && c < n) ? ws.a[(int64_t)r * lda + c] : 0.0;
-        }
-  };
-  const int32_t tc_end = min(nt, (ts + 1) * kS2Strip);
-  double old[2][2][4], nold[2][2][4];
-  load_old(k0 + ts * kS2Strip * 64, old);
-  for (int32_t tc = ts * kS2Strip; tc < tc_end; ++tc) {
-    const int32_t cb = k0 + tc * 64;
-    __syncthreads();  // the previous tile's reads of sb are done
-    for (int idx = threadIdx.x; idx < 64 * kS2K; idx += 256) {
-      const int o = idx / kS2K, k = idx % kS2K, q = k & (kNB - 1);
-      const int32_t c = cb + o;
-      const double *src_b = (k < kNB) ? ws.W : ws.V;  // B row c: [W_c | V_c]
-      sb[o][k] = (c < n && q < nb) ? src_b[(int64_t)c * kNB + q] : 0.0;
+      for (int h = 0; h < 8; ++h) av2[a][h] = src[h];
     }
-    __syncthreads();
-    // the next tile's old values are in flight while this tile is formed (the last
-    // tile reloads itself, unused)
-    load_old(tc + 1 < tc_end ? cb + 64 : cb, nold);
-    eig_f64x4 acc[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b) acc[a][b] = eig_f64x4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-    for (int ks = 0; ks < kS2K / 4; ++ks) {
-      const int k = 4 * ks + (lane >> 4);
-      double av[2], bv[2];
-#pragma unroll
-      for (int a = 0; a < 2; ++a) av[a] = sa[32 * wr + 16 * a + (lane & 15)][k];
-#pragma unroll
-      for (int b = 0; b < 2; ++b) bv[b] = sb[32 * wc + 16 * b + (lane & 15)][k];
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
-    }
-    // accumulator layout: lane l, element e -> row l/16 + 4e, column l%16
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int32_t r = rb + 32 * wr + 16 * a + (lane >> 4) + 4 * e;
-          const int32_t c = cb + 32 * wc + 16 * b + (lane & 15);
-          if (r < n && c < n) ws.a[(int64_t)r * lda + c] = old[a][b][e] - acc[a][b][e];
-        }
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) old[a][b][e] = nold[a][b][e];
   }
+  __builtin_amdgcn_sched_barrier(0);  // every load goes out before the LDS stores wait
+  s2_store(sb, cb, n, nb, rbv);
+  // panel columns at or past nb take no part (zero in A as in B)
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+      const int q = 16 * (g & 1) + 2 * h;
+      av2[a][h].x = q < nb ? av2[a][h].x : 0.0;
+      av2[a][h].y = q + 1 < nb ? av2[a][h].y : 0.0;
+    }
+  __syncthreads();
+  // the tile's old values (row, column of lane's accumulator elements) load while the
+  // products are formed; indices past n are clamped, and those elements are never stored
+  double old[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int32_t r = min(rb + 32 * wr + 16 * a + (lane >> 4) + 4 * e, n - 1);
+        const int32_t c = min(cb + 32 * wc + 16 * b + (lane & 15), n - 1);
+        old[a][b][e] = ws.a[(int64_t)r * lda + c];
+      }
+  __builtin_amdgcn_sched_barrier(0);  // the old values are in flight before the first product
+  eig_f64x4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = eig_f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int ks = 0; ks < 16; ++ks) {
+    double bv[2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) bv[b] = sb[32 * wc + 16 * b + (lane & 15)][16 * g + ks];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const double av = (ks & 1) ? av2[a][ks >> 1].y : av2[a][ks >> 1].x;
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+        acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[b], acc[a][b], 0, 0, 0);
+    }
+  }
+  // accumulator layout: lane l, element e -> row l/16 + 4e, column l%16
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int32_t r = rb + 32 * wr + 16 * a + (lane >> 4) + 4 * e;
+        const int32_t c = cb + 32 * wc + 16 * b + (lane & 15);
+        if (r < n && c < n) ws.a[(int64_t)r * lda + c] = old[a][b][e] - acc[a][b][e];
+      }
 }
 
 // ---------------------------------------------------------------- eigenvalues
@@ -1017,36 +1047,79 @@ __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec
 }
 
 // ---------------------------------------------------------------- back-transform
-// One block per panel p (reflector rows c0 = p*kNB .. c0+nb-1, v_q in A[c0+q][c0+q+1:],
-// v_q[c0+q+1] = 1): G = V^T V over all rows, then T (dlarft forward, columnwise:
-// T[0:j, j] = -tau_j T[0:j, 0:j] G[0:j, j]) into Tall[p].
+// Panel p holds reflector rows c0 = p*kNB .. c0+nb-1 (v_q in A[c0+q][c0+q+1:],
+// v_q[c0+q+1] = 1).  k_bt_gram forms G = V^T V over the panel's rows in kBtSplit
+// row ranges (one block each, partial G into part1), k_bt_prep sums a panel's partials
+// in range order and forms T (dlarft forward, columnwise:
+// T[0:j, j] = -tau_j T[0:j, 0:j] G[0:j, j]) into Tall[p].  part1 holds
+// (p_last + 1) * kBtSplit * kNB^2 doubles: 16 KNB^2 per 32 matrix rows, within its
+// 512 doubles per row.
 constexpr int kBtRows = 32;
-__global__ __launch_bounds__(256) void k_bt_prep(EigWs ws, int32_t n, int32_t last) {
-  __shared__ double sv[kNB][kBtRows + 1];
-  __shared__ double G[kNB][kNB + 1], T[kNB][kNB + 1];
-  const int32_t p = blockIdx.x;
+constexpr int kBtSplit = 16;   // row ranges per panel in k_bt_gram
+constexpr int kBtGRows = 128;  // rows staged per round in k_bt_gram
+__device__ __forceinline__ int32_t bt_gram_rows(int32_t n, int32_t c0) {
+  const int32_t rows = n - c0 - 1;  // reflector rows c0 + 1 .. n - 1
+  return (rows + kBtSplit * kBtGRows - 1) / (kBtSplit * kBtGRows) * kBtGRows;
+}
+__global__ __launch_bounds__(256) void k_bt_gram(EigWs ws, int32_t n, int32_t last) {
+  __shared__ double sv[kNB][kBtGRows + 1];
+  const int32_t p = blockIdx.x, s = blockIdx.y;
   const int32_t c0 = p * kNB;
   const int32_t nb = min(kNB, last - c0 + 1);
+  const int32_t per = bt_gram_rows(n, c0);
+  const int32_t r_beg = c0 + 1 + s * per, r_end = min(n, r_beg + per);
+  if (r_beg >= r_end) return;
   const int32_t lda = ws.lda;
-  // thread owns G entries idx = t + 256 k of the 32x32 grid
-  double g[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int32_t rb = c0 + 1; rb < n; rb += kBtRows) {
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < kNB * kBtRows; idx += 256) {
-      const int q = idx / kBtRows, i = idx % kBtRows;
-      const int32_t r = rb + i, cq = c0 + q;
-      sv[q][i] = (q < nb && r < n && r > cq) ? ws.a[(int64_t)cq * lda + r] : 0.0;
+  // element k of a thread: reflector q = (t + 256 k) / 128, row offset (t + 256 k) % 128;
+  // indices are clamped so every load is unconditional, and masked when staged
+  auto load = [&](int32_t rb, double (&v)[16]) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int idx = threadIdx.x + 256 * k, q = idx / kBtGRows, i = idx % kBtGRows;
+      const int32_t cq = c0 + min(q, nb - 1), r = min(rb + i, n - 1);
+      v[k] = ws.a[(int64_t)cq * lda + r];
+    }
+  };
+  double g[4] = {0.0, 0.0, 0.0, 0.0}, cur[16], nxt[16];
+  load(r_beg, cur);
+  for (int32_t rb = r_beg; rb < r_end; rb += kBtGRows) {
+    __syncthreads();  // the previous round's reads of sv are done
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int idx = threadIdx.x + 256 * k, q = idx / kBtGRows, i = idx % kBtGRows;
+      const int32_t r = rb + i;
+      sv[q][i] = (q < nb && r < r_end && r > c0 + q) ? cur[k] : 0.0;
     }
     __syncthreads();
+    if (rb + kBtGRows < r_end) load(rb + kBtGRows, nxt);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int idx = threadIdx.x + 256 * k;
       const int q1 = idx / kNB, q2 = idx % kNB;
-      double s = 0.0;
-      for (int i = 0; i < kBtRows; ++i) s += sv[q1][i] * sv[q2][i];
-      g[k] += s;
+      double a = 0.0;
+#pragma unroll 8
+      for (int i = 0; i < kBtGRows; ++i) a += sv[q1][i] * sv[q2][i];
+      g[k] += a;
     }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) cur[k] = nxt[k];
   }
+  double *out = ws.part1 + ((int64_t)p * kBtSplit + s) * (kNB * kNB);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) out[threadIdx.x + 256 * k] = g[k];
+}
+__global__ __launch_bounds__(256) void k_bt_prep(EigWs ws, int32_t n, int32_t last) {
+  __shared__ double G[kNB][kNB + 1], T[kNB][kNB + 1];
+  const int32_t p = blockIdx.x;
+  const int32_t c0 = p * kNB;
+  const int32_t nb = min(kNB, last - c0 + 1);
+  const int32_t per = bt_gram_rows(n, c0);
+  const int32_t nsp = (n - c0 - 1 + per - 1) / per;  // row ranges holding rows
+  const double *src = ws.part1 + (int64_t)p * kBtSplit * (kNB * kNB);
+  double g[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int32_t s = 0; s < nsp; ++s)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) g[k] += src[(int64_t)s * (kNB * kNB) + threadIdx.x + 256 * k];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int idx = threadIdx.x + 256 * k;
@@ -1252,8 +1325,8 @@ static void launch_tridiag(EigWs ws, int32_t n, int32_t clock_khz, hipStream_t s
     // panel end: update the trailing matrix before the next panel's first symv
     if (j == kNB - 1 && c + 2 < n) {
       const int32_t k1 = c + 1;
-      const int32_t nt = (n - k1 + 63) / 64, ns = (nt + kS2Strip - 1) / kS2Strip;
-      hipLaunchKernelGGL(k_syr2k, dim3((unsigned)(nt * ns)), dim3(256), 0, st, ws, n, k1, kNB);
+      const int32_t nt = (n - k1 + 63) / 64;
+      hipLaunchKernelGGL(k_syr2k, dim3((unsigned)(nt * nt)), dim3(256), 0, st, ws, n, k1, kNB);
     }
   }
 }
@@ -1398,8 +1471,11 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
     hipLaunchKernelGGL(k_invit, dim3(1), dim3(256), 0, st, ws, n, nvec, 0, (int32_t)kInvAll, 2);
     tend("invit");
     tbeg();
-    if (last >= 0)
+    if (last >= 0) {
+      hipLaunchKernelGGL(k_bt_gram, dim3((unsigned)(p_last + 1), kBtSplit), dim3(256), 0, st, ws, n,
+                         last);
       hipLaunchKernelGGL(k_bt_prep, dim3((unsigned)(p_last + 1)), dim3(256), 0, st, ws, n, last);
+    }
     EIG_TRY(back_transform(nvec));
   } else if (nvec > kMaxVec) {
     // the T-basis vectors of all batches (vector j at j*N) and the CGS coefficients
@@ -1411,8 +1487,11 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
     EIG_HIP(ctx, hipMemcpyAsync(sc, ws.sc, sizeof(sc), hipMemcpyDeviceToHost, st));
     EIG_HIP(ctx, hipStreamSynchronize(st));
     const double tol = 1e-3 * sc[S_TNORM];  // the cluster criterion of k_invit
-    if (last >= 0)
+    if (last >= 0) {
+      hipLaunchKernelGGL(k_bt_gram, dim3((unsigned)(p_last + 1), kBtSplit), dim3(256), 0, st, ws, n,
+                         last);
       hipLaunchKernelGGL(k_bt_prep, dim3((unsigned)(p_last + 1)), dim3(256), 0, st, ws, n, last);
+    }
     for (int32_t j0 = 0; j0 < nvec; j0 += kMaxVec) {
       const int32_t nb = std::min<int32_t>(kMaxVec, nvec - j0);
       int32_t c0 = j0;  // earlier vectors within tol of eigenvalue j0 (descending order)

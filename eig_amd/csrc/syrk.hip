@@ -390,55 +390,108 @@ __global__ __launch_bounds__(256, 2) void k_syrk_bf16(
 
 // ---------------------------------------------------------------- reduction
 // trace of the unscaled matrix from the diagonal tiles (one block, fixed order)
-__global__ void k_trace(const double *__restrict__ part, int32_t P, int32_t nsplit,
-                        int32_t nrows, double *__restrict__ tr) {
-  __shared__ double red[256];
-  double s = 0.0;
-  for (int32_t i = threadIdx.x; i < nrows; i += blockDim.x) {
-    int32_t ti = i / kTileS, r = i % kTileS;
-    int32_t p = ti * (ti + 1) / 2 + ti;
-    double v = 0.0;
-    for (int32_t sp = 0; sp < nsplit; ++sp)
-      v += part[((int64_t)sp * P + p) * (kTileS * kTileS) + r * kTileS + r];
-    s += v;
+// trace of the summed tiles, per diagonal tile: block ti sums rows ti*128 .. ti*128+127
+// (each row's splits in split order, four splits' loads in flight at once) into trp[ti]
+__global__ __launch_bounds__(kTileS) void k_trace(const double *__restrict__ part, int32_t P,
+                                                  int32_t nsplit, int32_t nrows,
+                                                  double *__restrict__ trp) {
+  __shared__ double red[kTileS];
+  const int32_t ti = blockIdx.x, r = threadIdx.x, i = ti * kTileS + r;
+  const int64_t stride = (int64_t)P * (kTileS * kTileS);
+  const int32_t p = ti * (ti + 1) / 2 + ti;
+  const double *d = part + (int64_t)p * (kTileS * kTileS) + r * kTileS + r;
+  double v = 0.0;
+  if (i < nrows) {
+    int32_t sp = 0;
+    for (; sp + 3 < nsplit; sp += 4) {
+      const double u0 = d[sp * stride], u1 = d[(sp + 1) * stride], u2 = d[(sp + 2) * stride],
+                   u3 = d[(sp + 3) * stride];
+      v += u0;
+      v += u1;
+      v += u2;
+      v += u3;
+    }
+    for (; sp < nsplit; ++sp) v += d[sp * stride];
   }
-  red[threadIdx.x] = s;
+  red[r] = v;
+  __syncthreads();
+  for (int o = kTileS / 2; o > 0; o >>= 1) {
+    if (r < o) red[r] += red[r + o];
+    __syncthreads();
+  }
+  if (r == 0) trp[ti] = red[0];
+}
+
+// the trace from the per-tile sums trp[0 .. nt-1], in a fixed order (the same bits in
+// every block); blocks of 256 threads, nt <= 256 * 8
+__device__ inline double trace_total(const double *__restrict__ trp, int32_t nt, double *red) {
+  double v = 0.0;
+  for (int32_t t = threadIdx.x; t < nt; t += 256) v += trp[t];
+  red[threadIdx.x] = v;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) tr[0] = red[0];
+  return red[0];
 }
 
 // full symmetric matrix, scaled by 1/y with y = trace/(nrows-1): one block per tile
 // pair sums its splits in split order (coalesced tile reads) and writes the tile
-// and, through LDS in 32-row slabs, its transpose.
+// and, through LDS in 32-row slabs, its transpose.  Every block forms the trace from
+// the per-tile sums; block 0 also stores it in tr[0] for the host.
 __global__ __launch_bounds__(256) void k_unpack_scale(const double *__restrict__ part, int32_t P,
                                                       int32_t nsplit, int32_t nrows,
-                                                      const double *__restrict__ tr,
+                                                      double *__restrict__ tr,
                                                       double *__restrict__ xtx) {
   __shared__ double sl[32][kTileS + 1];
+  __shared__ double red[256];
   const int32_t p = blockIdx.x;
   int32_t ti, tj;
   pair_to_tiles(p, ti, tj);
-  const double inv = 1.0 / (tr[0] / (double)(nrows - 1));
+  const double trace = trace_total(tr + 1, (nrows + kTileS - 1) / kTileS, red);
+  if (p == 0 && threadIdx.x == 0) tr[0] = trace;
+  const double inv = 1.0 / (trace / (double)(nrows - 1));
   const int64_t n = nrows;
   const int t = threadIdx.x;
   for (int r0 = 0; r0 < kTileS; r0 += 32) {
     // slab rows r0..r0+31 of the tile: thread t covers columns t & 127, rows t >> 7 + 2k
     __syncthreads();
+    // the splits are summed in split order, two splits' 32 loads in flight at once
+    const double *base = part + (int64_t)p * (kTileS * kTileS) + (r0 + (t >> 7)) * kTileS + (t & 127);
+    const int64_t stride = (int64_t)P * (kTileS * kTileS);
+    double v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = 0.0;
+    int32_t sp = 0;
+    for (; sp + 1 < nsplit; sp += 2) {
+      const double *b0 = base + sp * stride, *b1 = b0 + stride;
+      double u0[16], u1[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        u0[k] = b0[2 * k * kTileS];
+        u1[k] = b1[2 * k * kTileS];
+      }
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        v[k] += u0[k];
+        v[k] += u1[k];
+      }
+    }
+    if (sp < nsplit) {
+      const double *b0 = base + sp * stride;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) v[k] += b0[2 * k * kTileS];
+    }
+#pragma unroll
     for (int k = 0; k < 16; ++k) {
       const int rr = (t >> 7) + 2 * k, c = t & 127;
-      const int64_t off = (int64_t)(r0 + rr) * kTileS + c;
-      double v = 0.0;
-      for (int32_t sp = 0; sp < nsplit; ++sp) v += part[((int64_t)sp * P + p) * (kTileS * kTileS) + off];
-      v *= inv;
-      sl[rr][c] = v;
+      const double val = v[k] * inv;
+      sl[rr][c] = val;
       const int64_t gi = (int64_t)ti * kTileS + r0 + rr, gj = (int64_t)tj * kTileS + c;
       // lower triangle only: the upper is the transpose of it, so the matrix is
       // exactly symmetric (a diagonal tile's upper half is not used)
-      if (gi < n && gj <= gi) xtx[gi * n + gj] = v;
+      if (gi < n && gj <= gi) xtx[gi * n + gj] = val;
     }
     __syncthreads();
     // transpose: row (tile column c) of the slab's 32 values -> xtx[tj*128 + c][ti*128 + r0 ..]
@@ -529,7 +582,8 @@ int eig_run_syrk(eig_ctx *ctx, int form, int bf16_block, double *trace_scale) {
   }
   int32_t nsplit = (int32_t)((M + chunk - 1) / chunk);
   size_t tile = (size_t)kTileS * kTileS;
-  EIG_TRY(eig_ensure(ctx, ctx->part, sizeof(double) * (tile * P * nsplit + 64)));
+  // after the partial tiles: the trace, then the T per-tile diagonal sums
+  EIG_TRY(eig_ensure(ctx, ctx->part, sizeof(double) * (tile * P * nsplit + 64 + T)));
   EIG_TRY(eig_ensure(ctx, ctx->xtx, sizeof(double) * (size_t)n * n));
   double *d_tr = (double *)ctx->part.p + tile * P * nsplit;
   unsigned grid = (unsigned)(P * nsplit);
@@ -548,11 +602,10 @@ int eig_run_syrk(eig_ctx *ctx, int form, int bf16_block, double *trace_scale) {
   EIG_HIP(ctx, hipGetLastError());
   eig_time_end(ctx, "syrk");
   eig_time_begin(ctx);
-  hipLaunchKernelGGL(k_trace, dim3(1), dim3(256), 0, ctx->stream, (const double *)ctx->part.p,
-                     P, nsplit, n, d_tr);
+  hipLaunchKernelGGL(k_trace, dim3((unsigned)T), dim3(kTileS), 0, ctx->stream,
+                     (const double *)ctx->part.p, P, nsplit, n, d_tr + 1);
   hipLaunchKernelGGL(k_unpack_scale, dim3((unsigned)P), dim3(256), 0, ctx->stream,
-                     (const double *)ctx->part.p, P, nsplit, n, (const double *)d_tr,
-                     (double *)ctx->xtx.p);
+                     (const double *)ctx->part.p, P, nsplit, n, d_tr, (double *)ctx->xtx.p);
   EIG_HIP(ctx, hipGetLastError());
   double tr = 0.0;
   EIG_HIP(ctx, hipMemcpyAsync(&tr, d_tr, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
