@@ -1160,20 +1160,43 @@ __global__ __launch_bounds__(256) void k_bt_step(EigWs ws, int32_t n, int32_t p,
   const int32_t c0n = c0 + kNB;  // previous (higher) panel
   const int32_t nbn = apply_prev ? min(kNB, last - c0n + 1) : 0;
   // every operand goes to LDS in one round of loads: Z, the block's rows of X, and
-  // both panels' reflector entries over those rows
+  // both panels' reflector entries over those rows.  All loads are issued before the
+  // first LDS store (indices clamped into range; entries outside are stored as zero).
+  constexpr int kZU = kNB * kMaxVec / 256, kXU = kMaxVec * kBtRows / 256;
+  constexpr int kVU = kNB * kBtRows / 256;
+  double zv[kZU], xv[kXU], vv[kVU], vnv[kVU] = {};
+  const int t = threadIdx.x;
   if (apply_prev) {
-    for (int idx = threadIdx.x; idx < kNB * nvec; idx += 256)
-      sz[idx / nvec][idx % nvec] = ws.Z[(idx / nvec) * kMaxVec + (idx % nvec)];
+#pragma unroll
+    for (int u = 0; u < kZU; ++u) zv[u] = ws.Z[t + 256 * u];  // the whole kNB x kMaxVec block
   }
-  for (int idx = threadIdx.x; idx < nvec * kBtRows; idx += 256) {
-    const int o = idx / kBtRows, i = idx % kBtRows;
-    sx[o][i] = (i < rows) ? ws.X[(int64_t)o * n + rbase + i] : 0.0;
+#pragma unroll
+  for (int u = 0; u < kXU; ++u) {
+    const int idx = t + 256 * u, o = min(idx / kBtRows, nvec - 1), i = min(idx % kBtRows, rows - 1);
+    if (256 * u < nvec * kBtRows) xv[u] = ws.X[(int64_t)o * n + rbase + i];
   }
-  for (int idx = threadIdx.x; idx < kNB * kBtRows; idx += 256) {
-    const int q = idx / kBtRows, i = idx % kBtRows;
-    const int32_t r = rbase + i, cq = c0 + q, cqn = c0n + q;
-    sv[q][i] = (q < nb && i < rows && r > cq) ? ws.a[(int64_t)cq * lda + r] : 0.0;
-    svn[q][i] = (q < nbn && i < rows && r > cqn) ? ws.a[(int64_t)cqn * lda + r] : 0.0;
+#pragma unroll
+  for (int u = 0; u < kVU; ++u) {
+    const int idx = t + 256 * u, q = idx / kBtRows, i = min(idx % kBtRows, rows - 1);
+    vv[u] = ws.a[(int64_t)(c0 + min(q, nb - 1)) * lda + rbase + i];
+    if (nbn > 0) vnv[u] = ws.a[(int64_t)(c0n + min(q, nbn - 1)) * lda + rbase + i];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  if (apply_prev) {
+#pragma unroll
+    for (int u = 0; u < kZU; ++u) szbuf[t + 256 * u] = zv[u];
+  }
+#pragma unroll
+  for (int u = 0; u < kXU; ++u) {
+    const int idx = t + 256 * u, o = idx / kBtRows, i = idx % kBtRows;
+    if (256 * u < nvec * kBtRows && o < nvec) sx[o][i] = i < rows ? xv[u] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < kVU; ++u) {
+    const int idx = t + 256 * u, q = idx / kBtRows, i = idx % kBtRows;
+    const int32_t r = rbase + i;
+    sv[q][i] = (q < nb && i < rows && r > c0 + q) ? vv[u] : 0.0;
+    svn[q][i] = (q < nbn && i < rows && r > c0n + q) ? vnv[u] : 0.0;
   }
   __syncthreads();
   if (apply_prev) {
@@ -1205,11 +1228,14 @@ __global__ __launch_bounds__(1024) void k_bt_reduce(EigWs ws, int32_t p, int32_t
                                                     int32_t nvec, int32_t nblk) {
   __shared__ double acc[32][kNB + 1];
   __shared__ double col[kNB];
+  __shared__ double sT[kNB][kNB + 1];
   const int o = blockIdx.x;
   const int nd = kNB * nvec;
   const int32_t nb = min(kNB, last - p * kNB + 1);
   const int k = threadIdx.x % kNB, g = threadIdx.x / kNB;
+  const double tv = ws.Tall[(int64_t)p * kNB * kNB + threadIdx.x];  // T_p, one entry each
   acc[g][k] = sum_strided(ws.part1 + k * nvec + o, nd, g, 32, nblk);
+  sT[g][k] = tv;
   __syncthreads();
   if (threadIdx.x < kNB) {
     double s = acc[0][k];
@@ -1219,9 +1245,8 @@ __global__ __launch_bounds__(1024) void k_bt_reduce(EigWs ws, int32_t p, int32_t
   __syncthreads();
   if (threadIdx.x < kNB) {
     const int q = threadIdx.x;
-    const double *T = ws.Tall + (int64_t)p * kNB * kNB;
     double s = 0.0;
-    for (int kk = q; kk < nb; ++kk) s += T[q * kNB + kk] * col[kk];
+    for (int kk = q; kk < nb; ++kk) s += sT[q][kk] * col[kk];
     ws.Z[q * kMaxVec + o] = s;
   }
 }
