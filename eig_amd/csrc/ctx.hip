@@ -310,12 +310,16 @@ __global__ __launch_bounds__(256) void k_select(const uint8_t *__restrict__ geno
         }
       }
     } else {
+      // the word's 16 sample indices are loaded together (clamped; past nrows unused)
+      int32_t kk[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) kk[i] = xindex[min(st * kTileS + ww * 16 + i, nrows - 1)];
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
         const int32_t j = st * kTileS + ww * 16 + i;
         uint32_t code = 3;
         if (j < nrows) {
-          const int32_t k = xindex[j];
+          const int32_t k = kk[i];
           code = (row[k >> 2] >> ((3 - (k & 3)) << 1)) & 3u;
         }
         word |= code << (2 * i);

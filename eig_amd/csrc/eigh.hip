@@ -64,9 +64,40 @@ struct EigWs {
   double *Tall;           // panels x kNB x kNB
 };
 
-// sum over a 256-thread block; result valid in every thread
+// Exchange of a double with lane (lane ^ o) of the wave, for o a power of two below 64
+// that is a constant once the caller's loop is unrolled: o = 1, 2 and 8 by DPP (quad
+// permutes; a rotation by 8 within a 16-lane row), 4 and 16 by ds_swizzle (xor within
+// 32 lanes), 32 by ds_bpermute.  The values moved are those of __shfl_xor(v, o), so
+// every sum keeps its bits; DPP and swizzles skip the LDS address round trip.
+__device__ __forceinline__ int xlane32(int v, int o) {
+  switch (o) {
+    case 1: return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
+    case 2: return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);   // quad_perm 2,3,0,1
+    case 8: return __builtin_amdgcn_mov_dpp(v, 0x128, 0xF, 0xF, false);  // row_ror:8
+    case 4: return __builtin_amdgcn_ds_swizzle(v, (4 << 10) | 0x1F);
+    case 16: return __builtin_amdgcn_ds_swizzle(v, (16 << 10) | 0x1F);
+    default: return __shfl_xor(v, o, 64);
+  }
+}
+__device__ __forceinline__ double xlane(double v, int o) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)xlane32((int)(uint32_t)u, o);
+  const uint32_t hi = (uint32_t)xlane32((int)(uint32_t)(u >> 32), o);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+// lane 0's value in every lane (all lanes of the wave active)
+__device__ __forceinline__ double lane0(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// sum over a 256-thread block; result valid in every thread.  The wave sums are trees
+// over lane ^ o (lane 0 receives from lane o at each level, as with a shift down)
 __device__ inline double block_sum(double v, double *red) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += xlane(v, o);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
@@ -132,12 +163,13 @@ __device__ inline int32_t wcol_blocks(int32_t n, int32_t c) {  // producer block
 // of pw[b][0] and pw[b][1]).
 __device__ inline void col_scalars_sums(double s0, double s1, double &tau, double &scale,
                                         double &beta) {
+#pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
-    s0 += __shfl_down(s0, o, 64);
-    s1 += __shfl_down(s1, o, 64);
+    s0 += xlane(s0, o);
+    s1 += xlane(s1, o);
   }
-  s0 = __shfl(s0, 0, 64);
-  s1 = __shfl(s1, 0, 64);
+  s0 = lane0(s0);
+  s1 = lane0(s1);
   const double alpha = s1, xn2 = s0;
   tau = 0.0;
   scale = 0.0;
@@ -282,12 +314,12 @@ __device__ inline void symv_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
   for (int o = R / 2; o > 0; o >>= 1) {
     const bool b = (lane & o) != 0;
 #pragma unroll
-    for (int k = 0; k < o; ++k) acc[k] = (b ? acc[k + o] : acc[k]) + __shfl_xor(b ? acc[k] : acc[k + o], o, 64);
+    for (int k = 0; k < o; ++k) acc[k] = (b ? acc[k + o] : acc[k]) + xlane(b ? acc[k] : acc[k + o], o);
   }
   {
     double a = acc[0];
 #pragma unroll
-    for (int o = R; o < 64; o <<= 1) a += __shfl_xor(a, o, 64);
+    for (int o = R; o < 64; o <<= 1) a += xlane(a, o);
     if (lane < R) red[wv][lane] = a;
   }
   __syncthreads();
@@ -400,7 +432,8 @@ __device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
   const double yc1 = kEarly ? yc1e : ws.y[c1];
   double yv = kEarly ? sum_rest(ws.ps, 1, threadIdx.x, 256, nps, ype)
                      : sum_strided(ws.ps, 1, threadIdx.x, 256, nps);
-  for (int o = 32; o > 0; o >>= 1) yv += __shfl_down(yv, o, 64);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) yv += xlane(yv, o);
   if (lane == 0) red[wv][0] = yv;
   pqr[wv][lane] = praw;
   __syncthreads();
@@ -433,10 +466,10 @@ __device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
       sy += vrow[k] * pq[kNB + q] + wrow[k] * pq[q];
       sbse += cc[q] * wrow[k] + cc[kNB + q] * vrow[k];
     }
-    sy += __shfl_xor(sy, 1, 64);
-    sbse += __shfl_xor(sbse, 1, 64);
-    sy += __shfl_xor(sy, 2, 64);
-    sbse += __shfl_xor(sbse, 2, 64);
+    sy += xlane(sy, 1);
+    sbse += xlane(sbse, 1);
+    sy += xlane(sy, 2);
+    sbse += xlane(sbse, 2);
     yy -= sy;
     base -= sbse;
   }
@@ -465,9 +498,10 @@ __device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
   double *pwn = ws.pw + (int64_t)(c1 & 1) * ws.maxw * kPW + (int64_t)blk * kPW;
   {
     double v0 = sub == 0 ? xl * xl : 0.0, v1 = (sub == 0 && inr && r == c1 + 1) ? x : 0.0;
+#pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-      v0 += __shfl_down(v0, o, 64);
-      v1 += __shfl_down(v1, o, 64);
+      v0 += xlane(v0, o);
+      v1 += xlane(v1, o);
     }
     if (lane == 0) {
       red[wv][1] = v0;
@@ -492,7 +526,7 @@ __device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
       const bool bsel = (lane & off) != 0;
 #pragma unroll
       for (int k = 0; k < half; ++k)
-        t[k] = (bsel ? t[k + half] : t[k]) + __shfl_xor(bsel ? t[k] : t[k + half], off, 64);
+        t[k] = (bsel ? t[k + half] : t[k]) + xlane(bsel ? t[k] : t[k + half], off);
     }
     const int i = (lane >> 2) & 15;
     const int comp = (i < 8) ? 8 * sub + i : kNB + 8 * sub + (i - 8);
