@@ -59,24 +59,36 @@ struct EigWs {
   int32_t maxw;
   double *eval;           // ascending
   double *X;              // nvec x n eigenvectors
-  double *lu;             // nvec x 5n scratch for inverse iteration
+  double *lu;             // n x 5 x kMaxVec LU factors of inverse iteration
   double *Z;              // kNB x kMaxVec
   double *Tall;           // panels x kNB x kNB
 };
 
 // Exchange of a double with lane (lane ^ o) of the wave, for o a power of two below 64
-// that is a constant once the caller's loop is unrolled: o = 1, 2 and 8 by DPP (quad
-// permutes; a rotation by 8 within a 16-lane row), 4 and 16 by ds_swizzle (xor within
-// 32 lanes), 32 by ds_bpermute.  The values moved are those of __shfl_xor(v, o), so
-// every sum keeps its bits; DPP and swizzles skip the LDS address round trip.
+// that is a constant once the caller's loop is unrolled, without the LDS crossbar of
+// ds_bpermute: o = 1, 2 and 8 by one DPP move (quad permutes; a rotation by 8 within a
+// 16-lane row), o = 4 by two DPP moves (row shifts by 4 either way) and a select, o = 16
+// and 32 by v_permlane16_swap / v_permlane32_swap of the value with itself and a
+// select.  The values moved are those of __shfl_xor(v, o), so every sum keeps its bits.
 __device__ __forceinline__ int xlane32(int v, int o) {
+  const int lane = threadIdx.x & 63;
   switch (o) {
     case 1: return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
     case 2: return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);   // quad_perm 2,3,0,1
+    case 4: {
+      const int dn = __builtin_amdgcn_mov_dpp(v, 0x114, 0xF, 0xF, false);  // row_shr:4
+      const int up = __builtin_amdgcn_mov_dpp(v, 0x104, 0xF, 0xF, false);  // row_shl:4
+      return (lane & 4) ? dn : up;
+    }
     case 8: return __builtin_amdgcn_mov_dpp(v, 0x128, 0xF, 0xF, false);  // row_ror:8
-    case 4: return __builtin_amdgcn_ds_swizzle(v, (4 << 10) | 0x1F);
-    case 16: return __builtin_amdgcn_ds_swizzle(v, (16 << 10) | 0x1F);
-    default: return __shfl_xor(v, o, 64);
+    case 16: {  // rows 1, 3 of the first operand swap with rows 0, 2 of the second
+      const auto r = __builtin_amdgcn_permlane16_swap((unsigned)v, (unsigned)v, false, false);
+      return (int)((lane & 16) ? r[0] : r[1]);
+    }
+    default: {  // lanes 32-63 of the first operand swap with lanes 0-31 of the second
+      const auto r = __builtin_amdgcn_permlane32_swap((unsigned)v, (unsigned)v, false, false);
+      return (int)((lane & 32) ? r[0] : r[1]);
+    }
   }
 }
 __device__ __forceinline__ double xlane(double v, int o) {
@@ -896,6 +908,15 @@ constexpr int kInvB = 16;  // recurrence steps per batch of loads
 // batch runs all stages in one launch; longer runs of vectors go in batches with the
 // Gram-Schmidt against earlier batches between a launch's solves and its GS.
 enum { kInvFactor = 1, kInvStart = 2, kInvSolve = 4, kInvGs = 8, kInvAll = 15 };
+// The LU factors are interleaved over the vectors, so that one recurrence step's
+// accesses of all lanes share a cache line: field f of step i of vector kv sits at
+// lu[(i * kLuF + f) * kMaxVec + kv] (f: 0 pivot reciprocal, 1 and 2 the U
+// superdiagonals, 3 multiplier, 4 pivot flag).
+constexpr int kLuF = 5;
+struct LuCol {
+  double *p;  // lu + f * kMaxVec + kv
+  __device__ double &operator[](int32_t i) const { return p[(int64_t)i * (kLuF * kMaxVec)]; }
+};
 __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec, int32_t j0,
                                                int32_t flags, int32_t iters) {
   __shared__ double red[4];
@@ -907,8 +928,8 @@ __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec
   if ((flags & kInvFactor) && tid < nvec) {
     const int kv = tid;
     const double lam = ws.eval[n - 1 - (j0 + kv)];
-    double *u0 = ws.lu + (int64_t)kv * 5 * n, *u1 = u0 + n, *u2 = u1 + n, *ml = u2 + n,
-           *pv = ml + n;  // u0 holds pivot reciprocals
+    const LuCol u0{ws.lu + kv}, u1{ws.lu + kMaxVec + kv}, u2{ws.lu + 2 * kMaxVec + kv},
+        ml{ws.lu + 3 * kMaxVec + kv}, pv{ws.lu + 4 * kMaxVec + kv};  // u0: pivot reciprocals
     double acur = ws.d[0] - lam, bcur = (n > 1) ? ws.e[0] : 0.0;
     // the recurrence is serial; its operands are loaded kInvB steps at a time, one
     // batch ahead of the batch being computed, so memory latency stays hidden
@@ -971,8 +992,8 @@ __global__ __launch_bounds__(256) void k_invit(EigWs ws, int32_t n, int32_t nvec
   for (int it = 0; it < iters; ++it) {
     if ((flags & kInvSolve) && tid < nvec) {
       const int kv = tid;
-      const double *u0 = ws.lu + (int64_t)kv * 5 * n, *u1 = u0 + n, *u2 = u1 + n, *ml = u2 + n,
-                   *pv = ml + n;
+      const LuCol u0{ws.lu + kv}, u1{ws.lu + kMaxVec + kv}, u2{ws.lu + 2 * kMaxVec + kv},
+          ml{ws.lu + 3 * kMaxVec + kv}, pv{ws.lu + 4 * kMaxVec + kv};
       double *x = ws.X + (int64_t)kv * n;
       double carry = x[0];
       // forward: P L; batch i0 reads x[i0+1 .. i0+kInvB] and writes x[i0 .. i0+kInvB-1],
@@ -1423,7 +1444,7 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
          o_tau = add(N), o_V = add(kNB * N), o_W = add(kNB * N), o_x = add(lda), o_y = add(N),
          o_sc = add(S_COUNT), o_p1 = add((size_t)maxblk * maxnv),
          o_ev = add(N),
-         o_X = add((size_t)kMaxVec * N), o_lu = add((size_t)kMaxVec * 5 * N),
+         o_X = add((size_t)kMaxVec * N), o_lu = add((size_t)kMaxVec * kLuF * N),
          o_Z = add(kNB * kMaxVec), o_T = add((size_t)npanel * kNB * kNB),
          o_pw = add((size_t)2 * maxw * kPW), o_ps = add((size_t)maxblk), o_sync = add(8);
   void *old = ctx->ework.p;
