@@ -801,7 +801,6 @@ __device__ inline int32_t sturm1(const double *__restrict__ d, const double *__r
   auto step = [&](double di, double ei) {
     double r = __builtin_amdgcn_rcp(q);
     r = fma(r, fma(-q, r, 1.0), r);
-    r = fma(r, fma(-q, r, 1.0), r);
     q = (di - x) - ei * r;
     if (q <= pivmin) { ++cnt; q = fmin(q, -pivmin); }
   };
