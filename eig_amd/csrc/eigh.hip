@@ -852,11 +852,39 @@ __device__ inline int32_t sturm1(const double *__restrict__ d, const double *__r
 // (One count per lane: two interleaved counts per lane cut the rounds by a fifth
 // but double each round's work, and the kernel is bound by that work.)
 constexpr int kMS = 16;
-__global__ __launch_bounds__(256) void k_multisect(EigWs ws, int32_t n) {
+// Shared first level: Sturm counts at kMsGrid evenly spaced interior points of the
+// Gershgorin interval, one per thread (each a count over the whole tridiagonal, like
+// one round of the multisection for kMsGrid / 16 eigenvalues).  Every eigenvalue then
+// starts from the grid interval that brackets it, kMsGrid + 1 times narrower than the
+// Gershgorin interval, which saves about four rounds.
+constexpr int kMsGrid = 65536;
+__device__ __forceinline__ double msgrid_x(double glo, double ghi, int32_t j) {
+  return glo + (ghi - glo) / (double)(kMsGrid + 1) * (double)(j + 1);
+}
+__global__ __launch_bounds__(256) void k_msgrid(EigWs ws, int32_t n, int32_t *__restrict__ gcnt) {
+  const int32_t j = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);  // kMsGrid threads
+  gcnt[j] = sturm1(ws.d, ws.e2, n, msgrid_x(ws.sc[S_GL], ws.sc[S_GU], j), ws.sc[S_PIVMIN]);
+}
+// gcnt: k_msgrid's counts, or null to start from the Gershgorin interval
+__global__ __launch_bounds__(256) void k_multisect(EigWs ws, int32_t n,
+                                                   const int32_t *__restrict__ gcnt) {
   const int32_t k = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kMS);
   const int sub = threadIdx.x % kMS;
   const bool active = k < n;
   double lo = ws.sc[S_GL], hi = ws.sc[S_GU];
+  if (gcnt != nullptr && active) {
+    // the grid interval bracketing eigenvalue k: a ends as a point (or the lower bound,
+    // a = -1) whose count is <= k, b as one (or the upper bound, b = kMsGrid) whose
+    // count is > k, adjacent -- a valid start whatever the counts' rounding
+    int32_t a = -1, b = kMsGrid;
+    while (b - a > 1) {
+      const int32_t m = (a + b) >> 1;
+      if (gcnt[m] <= k) a = m; else b = m;
+    }
+    const double glo = lo, ghi = hi;
+    if (a >= 0) lo = msgrid_x(glo, ghi, a);
+    if (b < kMsGrid) hi = msgrid_x(glo, ghi, b);
+  }
   const double pivmin = ws.sc[S_PIVMIN];
   const double eps = 2.220446049250313e-16;
   // absolute floor eps * ||T||_1, the default of LAPACK's dstebz (ABSTOL <= 0): an
@@ -1514,7 +1542,15 @@ int eig_run_eigh(eig_ctx *ctx, const double *d_a, int32_t n, int32_t nvec, doubl
   hipLaunchKernelGGL(k_gersh, dim3(1), dim3(256), 0, st, ws, n);
   {
     int64_t threads = (int64_t)n * kMS;
-    hipLaunchKernelGGL(k_multisect, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, ws, n);
+    // the grid's counts go to the inverse iteration's LU area, unused until then
+    // (kMaxVec * 5 * N doubles hold the kMsGrid counts from N = 103 on)
+    int32_t *gcnt = nullptr;
+    if ((int64_t)kMaxVec * kLuF * N * 2 >= kMsGrid && n >= 128) {
+      gcnt = (int32_t *)ws.lu;
+      hipLaunchKernelGGL(k_msgrid, dim3(kMsGrid / 256), dim3(256), 0, st, ws, n, gcnt);
+    }
+    hipLaunchKernelGGL(k_multisect, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, ws, n,
+                       (const int32_t *)gcnt);
   }
   EIG_HIP(ctx, hipGetLastError());
   tend("bisect");
