@@ -424,10 +424,13 @@ __device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
   {
     const int comp = lane;  // 0..31 V dots, 32..63 W dots
     const int q = comp & (kNB - 1);
+    // pairwise, so the chain is five additions long instead of 24
+    double st[3];
 #pragma unroll
     for (int t = 0; t < 3; ++t)
-#pragma unroll
-      for (int u = 0; u < 8; ++u) praw += lp[t][u];
+      st[t] = ((lp[t][0] + lp[t][1]) + (lp[t][2] + lp[t][3])) +
+              ((lp[t][4] + lp[t][5]) + (lp[t][6] + lp[t][7]));
+    praw = (st[0] + st[1]) + st[2];
     for (int32_t bb = wv + 96; bb < nw; bb += 4) praw += pwc[2 + comp + (int64_t)bb * kPW];
     praw = q < j ? praw : 0.0;
   }
@@ -459,12 +462,20 @@ __device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
   __syncthreads();
   // alpha = -tau/2 w.v with w.v = tau (y.v - 2 sum_q p_q q_q), and w_pre at row c+1
   // (same in every block).  pq and cc are zero from q = j on, so the sums run over
-  // the whole panel (same bits as stopping at j) with every LDS read issued at once.
-  double spq = 0.0, y1 = yc1;
+  // the whole panel.  Lane q (mod 32) forms term q and five exchanges (lane ^ 16 ..
+  // lane ^ 1) add the terms as a tree: the same bits in every lane, wave and block.
+  double spq, y1;
+  {
+    const int q = lane & (kNB - 1);
+    double tq = pq[q] * pq[kNB + q];
+    double ty = cc[q] * pq[kNB + q] + cc[kNB + q] * pq[q];
 #pragma unroll
-  for (int q = 0; q < kNB; ++q) {
-    spq += pq[q] * pq[kNB + q];
-    y1 -= cc[q] * pq[kNB + q] + cc[kNB + q] * pq[q];
+    for (int o = kNB / 2; o > 0; o >>= 1) {
+      tq += xlane(tq, o);
+      ty += xlane(ty, o);
+    }
+    spq = tq;
+    y1 = yc1 - ty;
   }
   const double alpha = -0.5 * tau * (tau * (ydv - 2.0 * spq));
   const double wc1 = tau * y1;
