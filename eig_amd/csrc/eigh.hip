@@ -139,11 +139,12 @@ __device__ inline void load8(const double *__restrict__ p, int64_t stride, int32
     v[u] = bb < end ? x : 0.0;
   }
 }
+__device__ __forceinline__ double sum8(const double (&v)[8]) {  // pairwise
+  return ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+}
 __device__ inline double sum_rest(const double *__restrict__ p, int64_t stride, int32_t start,
                                   int32_t step, int32_t end, const double (&v0)[8]) {
-  double s = 0.0;
-#pragma unroll
-  for (int u = 0; u < 8; ++u) s += v0[u];
+  double s = sum8(v0);
   for (int32_t b = start + 8 * step; b < end; b += 8 * step) {
     double v[8];
 #pragma unroll
@@ -151,8 +152,7 @@ __device__ inline double sum_rest(const double *__restrict__ p, int64_t stride, 
       const int32_t bb = b + u * step;
       v[u] = bb < end ? p[(int64_t)bb * stride] : 0.0;
     }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) s += v[u];
+    s += sum8(v);
   }
   return s;
 }
@@ -482,13 +482,16 @@ __device__ inline void wcol_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
   // the row's panel corrections: each thread sums its quarter, two exchanges add the
   // quarters (the same bits in all four threads of the row)
   {
-    double sy = 0.0, sbse = 0.0;
+    double py[8], pb[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const int q = 8 * sub + k;
-      sy += vrow[k] * pq[kNB + q] + wrow[k] * pq[q];
-      sbse += cc[q] * wrow[k] + cc[kNB + q] * vrow[k];
+      py[k] = vrow[k] * pq[kNB + q] + wrow[k] * pq[q];
+      pb[k] = cc[q] * wrow[k] + cc[kNB + q] * vrow[k];
     }
+    // pairwise: three dependent additions instead of eight
+    double sy = ((py[0] + py[1]) + (py[2] + py[3])) + ((py[4] + py[5]) + (py[6] + py[7]));
+    double sbse = ((pb[0] + pb[1]) + (pb[2] + pb[3])) + ((pb[4] + pb[5]) + (pb[6] + pb[7]));
     sy += xlane(sy, 1);
     sbse += xlane(sbse, 1);
     sy += xlane(sy, 2);
