@@ -234,11 +234,13 @@ constexpr int kSymvRowsLong = 4;  // in k_symv (long columns): 106 VGPRs, 4 wave
 template <int R>
 __device__ inline void symv_part(const EigWs &ws, int32_t n, int32_t c, int32_t j, int32_t blk) {
   __shared__ double red[4][R];
-  __shared__ double yv[R];
   __shared__ double sscale;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int32_t lda = ws.lda;
   const int32_t r0 = c + 1 + blk * R;
+  // the block's own entries of x, for its rows of v at the end (loaded now, so no
+  // memory round trip follows the stream)
+  const double xown = (threadIdx.x < R && r0 + (int32_t)threadIdx.x < n) ? ws.xtmp[r0 + threadIdx.x] : 0.0;
   const int32_t a0 = (c + 1) & ~1;
   const int32_t npair = (((n + 1) & ~1) - a0) >> 1;
   const double2 *xp = (const double2 *)(ws.xtmp + a0);
@@ -335,27 +337,25 @@ __device__ inline void symv_part(const EigWs &ws, int32_t n, int32_t c, int32_t 
     if (lane < R) red[wv][lane] = a;
   }
   __syncthreads();
-  // per row: y, v into V[j] and A[c], and the row's y.v
-  if (threadIdx.x < R) {
-    const int rr = threadIdx.x;
-    const int32_t r = r0 + rr;
+  // per row (lanes 0 .. R-1 of wave 0): y, v into V[j] and A[c], and the row's y.v;
+  // the block's y.v partial is their sum over the wave (a tree of R/2 .. 1 exchanges)
+  if (wv == 0) {
     double val = 0.0;
-    if (r < n) {
-      const double vr = (r == c + 1) ? 1.0 : ws.xtmp[r] * scale;
-      const double yr = (red[0][rr] + red[1][rr]) + (red[2][rr] + red[3][rr]);
-      val = yr * vr;
-      store_dev(ws.y + r, yr);
-      ws.V[(int64_t)r * kNB + j] = vr;
-      ws.a[(int64_t)c * lda + r] = vr;
+    if (lane < R) {
+      const int rr = lane;
+      const int32_t r = r0 + rr;
+      if (r < n) {
+        const double vr = (r == c + 1) ? 1.0 : xown * scale;
+        const double yr = (red[0][rr] + red[1][rr]) + (red[2][rr] + red[3][rr]);
+        val = yr * vr;
+        store_dev(ws.y + r, yr);
+        ws.V[(int64_t)r * kNB + j] = vr;
+        ws.a[(int64_t)c * lda + r] = vr;
+      }
     }
-    yv[rr] = val;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double sum = 0.0;
 #pragma unroll
-    for (int rr = 0; rr < R; ++rr) sum += yv[rr];
-    store_dev(ws.ps + blk, sum);
+    for (int o = R / 2; o > 0; o >>= 1) val += xlane(val, o);
+    if (lane == 0) store_dev(ws.ps + blk, val);
   }
 }
 
