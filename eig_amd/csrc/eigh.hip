@@ -21,8 +21,9 @@
 //     k_syr2k applies A -= V W^T + W V^T to the trailing matrix.  The reflector of
 //     row c is left in A[c][c+1:], with tau[c].  The launch sequence for one n is
 //     captured once into a hipGraph and replayed.
-//  2. All eigenvalues of the tridiagonal T by 33-way multisection on Sturm counts
-//     (16 lanes per eigenvalue, two interleaved counts per lane).
+//  2. All eigenvalues of the tridiagonal T: Sturm counts on a shared grid of the
+//     Gershgorin interval, then 9-way multisection (8 lanes per eigenvalue, one
+//     count each).
 //  3. The top nvec eigenvectors of T by inverse iteration (LU with partial
 //     pivoting), Gram-Schmidt inside clusters of close eigenvalues.
 //  4. Back-transform X = Q Y in compact-WY form Q_p = I - V_p T_p V_p^T, last
@@ -861,11 +862,11 @@ __device__ inline int32_t sturm1(const double *__restrict__ d, const double *__r
   return cnt;
 }
 
-// 16 lanes per eigenvalue (k-th smallest): each round the 16 lanes evaluate 16
-// interior points of [lo, hi] and keep the sub-interval holding eigenvalue k.
-// (One count per lane: two interleaved counts per lane cut the rounds by a fifth
-// but double each round's work, and the kernel is bound by that work.)
-constexpr int kMS = 16;
+// kMS lanes per eigenvalue (k-th smallest): each round the lanes evaluate kMS interior
+// points of [lo, hi] and keep the sub-interval holding eigenvalue k.  One count per
+// lane; at N=5,000 (after the grid) 8 lanes took 2.52 ms, 4 lanes 3.16, 16 lanes 2.94
+// and 32 lanes 3.70: more lanes cut the rounds but add each round's work.
+constexpr int kMS = 8;
 // Shared first level: Sturm counts at kMsGrid evenly spaced interior points of the
 // Gershgorin interval, one per thread (each a count over the whole tridiagonal, like
 // one round of the multisection for kMsGrid / 16 eigenvalues).  Every eigenvalue then
@@ -908,7 +909,7 @@ __global__ __launch_bounds__(256) void k_multisect(EigWs ws, int32_t n,
   for (int it = 0; it < 96; ++it) {
     const double tol = 2.0 * eps * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin + atol;
     const bool done = (hi - lo <= tol);
-    // all 16 lanes of a group share lo/hi, so `done` is uniform within a group
+    // all kMS lanes of a group share lo/hi, so `done` is uniform within a group
     if (__all(done || !active)) break;
     const double w = (hi - lo) / (double)(kMS + 1);
     const double x = lo + w * (double)(sub + 1);
