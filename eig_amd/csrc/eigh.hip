@@ -27,9 +27,11 @@
 //  3. The top nvec eigenvectors of T by inverse iteration (LU with partial
 //     pivoting), Gram-Schmidt inside clusters of close eigenvalues.
 //  4. Back-transform X = Q Y in compact-WY form Q_p = I - V_p T_p V_p^T, last
-//     panel first; all T_p are formed up front in one launch, then per panel one
-//     kernel applies the previous panel and writes per-block dots V_p^T X, and a
-//     one-block kernel sums them in block order into Z_p = T_p V_p^T X.
+//     panel first; all T_p are formed up front (the Gram matrices V_p^T V_p over
+//     row ranges, then T_p per panel), then per panel one kernel applies the
+//     previous panel and writes per-block dots V_p^T X, and a kernel with one block
+//     per vector sums them in block order into Z_p = T_p V_p^T X.
+//  Wave-level sums exchange lanes by DPP and permlane swaps (xlane), not through LDS.
 #include <cmath>
 #include <cstring>
 #include <mutex>
