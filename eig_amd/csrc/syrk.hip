@@ -389,7 +389,6 @@ __global__ __launch_bounds__(256, 2) void k_syrk_bf16(
 }
 
 // ---------------------------------------------------------------- reduction
-// trace of the unscaled matrix from the diagonal tiles (one block, fixed order)
 // trace of the summed tiles, per diagonal tile: block ti sums rows ti*128 .. ti*128+127
 // (each row's splits in split order, four splits' loads in flight at once) into trp[ti]
 __global__ __launch_bounds__(kTileS) void k_trace(const double *__restrict__ part, int32_t P,
@@ -423,7 +422,7 @@ __global__ __launch_bounds__(kTileS) void k_trace(const double *__restrict__ par
 }
 
 // the trace from the per-tile sums trp[0 .. nt-1], in a fixed order (the same bits in
-// every block); blocks of 256 threads, nt <= 256 * 8
+// every block); blocks of 256 threads
 __device__ inline double trace_total(const double *__restrict__ trp, int32_t nt, double *red) {
   double v = 0.0;
   for (int32_t t = threadIdx.x; t < nt; t += 256) v += trp[t];
