@@ -335,6 +335,7 @@ def cfg3_leg(args, max_passes=6):
 
 
 def main():
+    global BF16_BLOCK  # set from --bf16-block; one_pass and the legs read it
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
@@ -358,14 +359,14 @@ def main():
     ap.add_argument("--fastmode", action="store_true",
                     help="instead of the XTX pass, time fastmode (K=10, L=20, I=10) on the "
                          "100,000 x 500,000 panel in both panel-product forms")
-    ap.add_argument("--bf16-block", type=int, default=256,
+    ap.add_argument("--bf16-block", type=int, default=BF16_BLOCK,
                     help="SNPs per FP32 accumulation block of the split-BF16 forms (a multiple "
                          "of 16; the headline line is quoted at 256)")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="first copy this many MiB device-to-device (the known-size copy "
                          "that scales the HBM byte counters of a --pmc run)")
     args = ap.parse_args()
-    globals()["BF16_BLOCK"] = args.bf16_block
+    BF16_BLOCK = args.bf16_block
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported",
                           "n_gpus": args.gpus}))
