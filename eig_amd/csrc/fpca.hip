@@ -168,6 +168,98 @@ __global__ __launch_bounds__(256) void k_fpca_xa(const uint32_t *__restrict__ se
   }
 }
 
+// The same Y = X A with the next sample stage's loads (the panel rows and each
+// lane's genotype words) issued before the current stage's MFMAs: the panel stage
+// goes global -> registers -> one of two LDS buffers, one barrier per stage.  For
+// CT <= 2, as k_fpca_xtb_pf.
+template <int CT>
+__global__ __launch_bounds__(256, 4) void k_fpca_xa_pf(const uint32_t *__restrict__ sel,
+                                                       int64_t M, int32_t nst,
+                                                       const double *__restrict__ lf,
+                                                       const double *__restrict__ A, int32_t lda,
+                                                       double *__restrict__ Y, int64_t ldy,
+                                                       int32_t C) {
+  constexpr int kAE = kKC * 16 * CT / 256;  // panel elements per thread per stage
+  constexpr int kStT = kTileS / kKC;        // stages per sample tile
+  __shared__ double sA[2][kKC][16 * CT];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int64_t ntiles = (M + 127) / 128;
+  const int32_t nstage = nst * kStT;
+  double ra[kAE];
+  uint2 rw[2];
+  int buf = 0;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t mb = tile * 128 + 32 * w;
+    int64_t r[2];
+    double lv[2][3];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      r[t] = mb + 16 * t + (lane & 15);
+      const bool ok = r[t] < M;
+#pragma unroll
+      for (int g = 0; g < 3; ++g) lv[t][g] = ok ? lf[4 * r[t] + g] : 0.0;
+    }
+    auto fetch = [&](int32_t q) {
+      const int32_t st = q / kStT, kc = (q % kStT) * kKC;
+      const double *src = A + (int64_t)(st * kTileS + kc) * lda;
+#pragma unroll
+      for (int j = 0; j < kAE; ++j) {
+        const int idx = tid + 256 * j;
+        ra[j] = src[(int64_t)(idx / (16 * CT)) * lda + idx % (16 * CT)];
+      }
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+        rw[t] = r[t] < M ? *(const uint2 *)(sel + ((int64_t)st * M + r[t]) * kTileWords + kc / 16)
+                         : make_uint2(~0u, ~0u);
+    };
+    f64x4 acc[2][CT];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int c = 0; c < CT; ++c) acc[t][c] = f64x4{0.0, 0.0, 0.0, 0.0};
+    fetch(0);
+    for (int32_t q = 0; q < nstage; ++q) {
+#pragma unroll
+      for (int j = 0; j < kAE; ++j) {
+        const int idx = tid + 256 * j;
+        sA[buf][idx / (16 * CT)][idx % (16 * CT)] = ra[j];
+      }
+      const uint32_t wd[2][2] = {{rw[0].x, rw[0].y}, {rw[1].x, rw[1].y}};
+      // the other buffer was last read before this barrier's previous instance
+      __syncthreads();
+      if (q + 1 < nstage) fetch(q + 1);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int ks = 0; ks < kKC / 4; ++ks) {
+        const int sh = 2 * ((ks & 3) * 4 + (lane >> 4));
+        double a[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          const uint32_t code = (wd[t][ks >> 2] >> sh) & 3u;
+          a[t] = code == 0 ? lv[t][0] : (code == 1 ? lv[t][1] : (code == 2 ? lv[t][2] : 0.0));
+        }
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+          const double b = sA[buf][4 * ks + (lane >> 4)][16 * c + (lane & 15)];
+          acc[0][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b, acc[0][c], 0, 0, 0);
+          acc[1][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1], b, acc[1][c], 0, 0, 0);
+        }
+      }
+      buf ^= 1;
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int c = 0; c < CT; ++c)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int64_t row = mb + 16 * t + (lane >> 4) + 4 * e;
+          const int col = 16 * c + (lane & 15);
+          if (row < M && col < C) Y[row * ldy + col] = acc[t][c][e];
+        }
+  }
+}
+
 // ---------------------------------------------------------------- partial X^T B
 // Persistent over (sample tile, SNP split) items; a block covers one sample tile
 // (4 waves x 32 samples) and SNPs [s*mchunk, min(M, (s+1)*mchunk)).  B: rows m,
@@ -231,6 +323,106 @@ __global__ __launch_bounds__(256) void k_fpca_xtb(const uint32_t *__restrict__ s
           acc[1][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1], b, acc[1][c], 0, 0, 0);
         }
       }
+    }
+    double *out = part + ((int64_t)s * npad + (int64_t)st * kTileS) * ldp;
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int c = 0; c < CT; ++c)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int row = 32 * w + 16 * t + (lane >> 4) + 4 * e;
+          out[(int64_t)row * ldp + 16 * c + (lane & 15)] = acc[t][c][e];
+        }
+  }
+}
+
+// The same partial X^T B with the next SNP stage's loads issued before the current
+// stage's MFMAs: the stage goes global -> registers -> one of two LDS buffers, so
+// one barrier per stage suffices and the load latency hides behind 8 MFMA steps.
+// For CT <= 2, whose prefetch registers fit beside the accumulators at four waves
+// per SIMD (wider instances spill there).
+template <int CT>
+__global__ __launch_bounds__(256, 4) void k_fpca_xtb_pf(const uint32_t *__restrict__ sel, int64_t M,
+                                                     int32_t nst, const double *__restrict__ lf,
+                                                     const double *__restrict__ B, int64_t ldb,
+                                                     int32_t C, int32_t S, int64_t mchunk,
+                                                     double *__restrict__ part, int32_t ldp) {
+  constexpr int kBE = kKC * 16 * CT / 256;  // B elements per thread per stage
+  static_assert(kKC * 4 == 128 && kKC * kTileWords / 2 == 128, "stage loads take 256 threads");
+  __shared__ double sB[2][kKC][16 * CT];
+  __shared__ double sL[2][kKC][4];
+  __shared__ uint32_t sW[2][kKC][kTileWords];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int64_t items = (int64_t)nst * S;
+  const int64_t npad = (int64_t)nst * kTileS;
+  double rb[kBE];
+  double rl = 0.0;
+  uint2 rw = make_uint2(~0u, ~0u);
+  int buf = 0;
+  for (int64_t it = blockIdx.x; it < items; it += gridDim.x) {
+    const int32_t st = (int32_t)(it % nst), s = (int32_t)(it / nst);
+    const int64_t m0 = (int64_t)s * mchunk, m1 = min(M, m0 + mchunk);
+    // threads 0..127: level (k, g) = (tid/4, tid%4); 128..255: uint2 q of SNP k's record
+    auto fetch = [&](int64_t mb) {
+#pragma unroll
+      for (int j = 0; j < kBE; ++j) {
+        const int idx = tid + 256 * j;
+        const int k = idx / (16 * CT), c = idx % (16 * CT);
+        const int64_t m = mb + k;
+        rb[j] = (m < m1 && c < C) ? B[m * ldb + c] : 0.0;
+      }
+      if (tid < 128) {
+        const int64_t m = mb + (tid >> 2);
+        rl = m < m1 ? lf[4 * m + (tid & 3)] : 0.0;
+      } else {
+        const int q = tid - 128;
+        const int64_t m = mb + (q >> 2);
+        rw = make_uint2(~0u, ~0u);
+        if (m < m1) rw = *(const uint2 *)(sel + ((int64_t)st * M + m) * kTileWords + (q & 3) * 2);
+      }
+    };
+    f64x4 acc[2][CT];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int c = 0; c < CT; ++c) acc[t][c] = f64x4{0.0, 0.0, 0.0, 0.0};
+    if (m0 < m1) fetch(m0);
+    for (int64_t mb = m0; mb < m1; mb += kKC) {
+#pragma unroll
+      for (int j = 0; j < kBE; ++j) {
+        const int idx = tid + 256 * j;
+        sB[buf][idx / (16 * CT)][idx % (16 * CT)] = rb[j];
+      }
+      if (tid < 128) {
+        sL[buf][tid >> 2][tid & 3] = rl;
+      } else {
+        const int q = tid - 128;
+        sW[buf][q >> 2][(q & 3) * 2] = rw.x;
+        sW[buf][q >> 2][(q & 3) * 2 + 1] = rw.y;
+      }
+      // the other buffer was last read before this barrier's previous instance
+      __syncthreads();
+      if (mb + kKC < m1) fetch(mb + kKC);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int ks = 0; ks < kKC / 4; ++ks) {
+        const int k = 4 * ks + (lane >> 4);
+        double a[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          const int sl = 32 * w + 16 * t + (lane & 15);  // sample within the tile
+          const uint32_t code = (sW[buf][k][sl >> 4] >> (2 * (sl & 15))) & 3u;
+          a[t] = sL[buf][k][code];
+        }
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+          const double b = sB[buf][k][16 * c + (lane & 15)];
+          acc[0][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b, acc[0][c], 0, 0, 0);
+          acc[1][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1], b, acc[1][c], 0, 0, 0);
+        }
+      }
+      buf ^= 1;
     }
     double *out = part + ((int64_t)s * npad + (int64_t)st * kTileS) * ldp;
 #pragma unroll
@@ -718,8 +910,11 @@ struct Plan {
 static int launch_xa(eig_ctx *ctx, const Plan &P, int CT, int grid, const double *A, int32_t lda,
                      double *Y, int64_t ldy, int32_t C, hipStream_t st) {
   FP_CTSWITCH(ctx, CT,
-              hipLaunchKernelGGL((k_fpca_xa<CT>), dim3(grid), dim3(256), 0, st, P.sel, P.M, P.nst,
-                                 P.lf, A, lda, Y, ldy, C));
+              if constexpr (CT <= 2) hipLaunchKernelGGL((k_fpca_xa_pf<CT>), dim3(grid), dim3(256),
+                                                        0, st, P.sel, P.M, P.nst, P.lf, A, lda, Y,
+                                                        ldy, C);
+              else hipLaunchKernelGGL((k_fpca_xa<CT>), dim3(grid), dim3(256), 0, st, P.sel, P.M,
+                                      P.nst, P.lf, A, lda, Y, ldy, C));
   return EIG_OK;
 }
 
@@ -729,8 +924,11 @@ static int launch_xtb(eig_ctx *ctx, const Plan &P, int CT, int grid, int S, cons
   const int64_t mchunk = ((P.M + S - 1) / S + kKC - 1) / kKC * kKC;
   const int32_t ldp = 16 * CT;
   FP_CTSWITCH(ctx, CT,
-              hipLaunchKernelGGL((k_fpca_xtb<CT>), dim3(grid), dim3(256), 0, st, P.sel, P.M, P.nst,
-                                 P.lf, B, ldb, C, S, mchunk, part, ldp));
+              if constexpr (CT <= 2) hipLaunchKernelGGL((k_fpca_xtb_pf<CT>), dim3(grid), dim3(256),
+                                                        0, st, P.sel, P.M, P.nst, P.lf, B, ldb, C,
+                                                        S, mchunk, part, ldp);
+              else hipLaunchKernelGGL((k_fpca_xtb<CT>), dim3(grid), dim3(256), 0, st, P.sel, P.M,
+                                      P.nst, P.lf, B, ldb, C, S, mchunk, part, ldp));
   const int64_t tot = (int64_t)P.npad * ldp;
   hipLaunchKernelGGL(k_fpca_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
                      (const double *)part, S, (int64_t)P.npad, ldp, scale, out, ldo);
