@@ -607,6 +607,124 @@ __global__ __launch_bounds__(256, 2) void k_fpca_xa_bf16(
   }
 }
 
+// The same Y = X A with the next sample tile's panel and genotype words loaded
+// before the current tile's MFMAs: the panel goes global -> registers -> one of two
+// LDS buffers, one barrier per tile.  For CT == 1, whose two panel buffers fit
+// beside the tables at two blocks per CU.
+template <int CT>
+__global__ __launch_bounds__(256, 2) void k_fpca_xa_bf16_pf(
+    const uint32_t *__restrict__ sel, int64_t M, int32_t nst, const uint32_t *__restrict__ lbf,
+    const uint16_t *__restrict__ Ah, const uint16_t *__restrict__ Al, int64_t ldt,
+    double *__restrict__ Y, int64_t ldy, int32_t C, int32_t blkt) {
+  constexpr int kPE = 2 * 32 * CT * 16 / 256;  // panel uint4 per thread per tile
+  __shared__ uint32_t tt[2][128 * 17 + 1];
+  __shared__ __attribute__((aligned(16))) uint16_t sA[2][2][32 * CT][128 + 8];  // [buf][hi/lo]
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63, h = lane >> 5, r = lane & 31;
+  const int64_t ntiles = (M + 127) / 128;
+  fp_u32x4 pa[kPE], wa, wb;
+  int buf = 0;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t mb = tile * 128;
+    __syncthreads();  // the previous tile's table reads are done
+    for (int e = t; e < 128 * 16; e += 256) {
+      const int sl = e >> 4, c = e & 15;
+      const int64_t m = mb + sl;
+      uint32_t e0 = 0u, e1 = 0u;
+      if (m < M) {
+        e0 = lbf[4 * m + (c & 3)];
+        e1 = lbf[4 * m + (c >> 2)];
+      }
+      tt[0][sl * 17 + c] = (e0 & 0xFFFFu) | (e1 << 16);
+      tt[1][sl * 17 + c] = (e0 >> 16) | (e1 & 0xFFFF0000u);
+    }
+    const int64_t m = mb + 32 * w + r;  // this lane's SNP
+    const uint32_t rowoff = (uint32_t)((32 * w + r) * 17);
+    auto fetch = [&](int32_t st) {
+#pragma unroll
+      for (int j = 0; j < kPE; ++j) {
+        const int q = t + 256 * j;
+        const int hl = q / (32 * CT * 16), rem = q % (32 * CT * 16), c = rem >> 4, ch = rem & 15;
+        pa[j] = *(const fp_u32x4 *)((hl ? Al : Ah) + (int64_t)c * ldt + (int64_t)st * kTileS + ch * 8);
+      }
+      // a SNP past M reads SNP M-1's words (in bounds) and takes all-missing codes
+      const uint32_t *ws = sel + ((int64_t)st * M + (m < M ? m : M - 1)) * kTileWords;
+      const fp_u32x4 x0 = *(const fp_u32x4 *)ws, x1 = *(const fp_u32x4 *)(ws + 4);
+      const fp_u32x4 none = {~0u, ~0u, ~0u, ~0u};
+      wa = m < M ? x0 : none;
+      wb = m < M ? x1 : none;
+    };
+    fp_f32x16 acc[CT];
+    double dacc[CT][16];
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        acc[ct][q] = 0.f;
+        dacc[ct][q] = 0.0;
+      }
+    auto promote = [&]() {
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          dacc[ct][q] += (double)acc[ct][q];
+          acc[ct][q] = 0.f;
+        }
+    };
+    fetch(0);
+    int32_t left = blkt;
+    for (int32_t st = 0; st < nst; ++st) {
+#pragma unroll
+      for (int j = 0; j < kPE; ++j) {
+        const int q = t + 256 * j;
+        const int hl = q / (32 * CT * 16), rem = q % (32 * CT * 16), c = rem >> 4, ch = rem & 15;
+        *(fp_u32x4 *)&sA[buf][hl][c][ch * 8] = pa[j];
+      }
+      const fp_u32x4 ca = wa, cb = wb;  // this tile's words; wa, wb take the next tile's
+      // the other buffer was last read before this barrier's previous instance
+      __syncthreads();
+      if (st + 1 < nst) fetch(st + 1);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {  // 16-sample steps of the tile
+        const uint32_t bits = (j < 4 ? ca[j & 3] : cb[j & 3]) >> (16 * h);
+        fp_u32x4 xh, xl;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t c = (bits >> (4 * q)) & 15u;
+          xh[q] = tt[0][rowoff + c];
+          xl[q] = tt[1][rowoff + c];
+        }
+        const fp_bf16x8 ah = __builtin_bit_cast(fp_bf16x8, xh), al = __builtin_bit_cast(fp_bf16x8, xl);
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+          const fp_bf16x8 bh = *(const fp_bf16x8 *)&sA[buf][0][32 * ct + r][16 * j + 8 * h];
+          const fp_bf16x8 bl = *(const fp_bf16x8 *)&sA[buf][1][32 * ct + r][16 * j + 8 * h];
+          acc[ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bl, acc[ct], 0, 0, 0);
+          acc[ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[ct], 0, 0, 0);
+          acc[ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[ct], 0, 0, 0);
+          acc[ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[ct], 0, 0, 0);
+        }
+      }
+      if (--left == 0) {
+        promote();
+        left = blkt;
+      }
+      buf ^= 1;
+    }
+    promote();
+    // accumulator: lane l, element q -> row 8(q>>2) + 4(l>>5) + (q&3), column l&31
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int64_t row = mb + 32 * w + 8 * (q >> 2) + 4 * h + (q & 3);
+        const int col = 32 * ct + r;
+        if (row < M && col < C) Y[row * ldy + col] = dacc[ct][q];
+      }
+  }
+}
+
 // Partial X^T B.  Items (sample tile, SNP split, 64-column group); a block covers one
 // sample tile (4 waves x 32 samples) x 64 columns over SNPs [s*mchunk, ...) (mchunk a
 // multiple of 64).  The genotype operand comes from the grouped layout through
@@ -712,7 +830,7 @@ __global__ __launch_bounds__(256, 2) void k_fpca_xtb_bf16(
       const uint32_t *th = tab[buf][0], *tl = tab[buf][1];
 #pragma unroll
       for (int s4 = 0; s4 < 4; ++s4) {
-        if (s4 == 2) {
+        if (s4 == 0) {
           lev_fetch(gn, lv);
           pan_fetch(gn);
           nxt = src[(int64_t)gn * (2 * kTileS)];
@@ -1200,7 +1318,7 @@ static int launch_xa_bf16(eig_ctx *ctx, const Plan &P, const uint32_t *lbf, uint
                      dim3(256), 0, st, A, (int64_t)P.npad, (int64_t)lda, C, (int64_t)P.npad, cxa,
                      th, tl);
   if (cxa == 32)
-    hipLaunchKernelGGL(k_fpca_xa_bf16<1>, dim3(grid), dim3(256), 0, st, P.sel, P.M, P.nst, lbf,
+    hipLaunchKernelGGL(k_fpca_xa_bf16_pf<1>, dim3(grid), dim3(256), 0, st, P.sel, P.M, P.nst, lbf,
                        (const uint16_t *)th, (const uint16_t *)tl, (int64_t)P.npad, Y, ldy, C, 4);
   else
     hipLaunchKernelGGL(k_fpca_xa_bf16<2>, dim3(grid), dim3(256), 0, st, P.sel, P.M, P.nst, lbf,
