@@ -725,20 +725,22 @@ __global__ __launch_bounds__(256, 2) void k_fpca_xa_bf16_pf(
   }
 }
 
-// Partial X^T B.  Items (sample tile, SNP split, 64-column group); a block covers one
-// sample tile (4 waves x 32 samples) x 64 columns over SNPs [s*mchunk, ...) (mchunk a
-// multiple of 64).  The genotype operand comes from the grouped layout through
+// Partial X^T B.  Items (sample tile, SNP split, column group of 32 NCT); a block covers
+// one sample tile (4 waves x 32 samples) x 32 NCT columns over SNPs [s*mchunk, ...)
+// (mchunk a multiple of 64).  The genotype operand comes from the grouped layout through
 // per-group tables of level pairs (the split-BF16 XTX's scheme); the panel operand
 // from Bh/Bl [column][SNP] (row stride ldt >= 64 * ceil(M/64)), staged per group.
 // part[s][i][c] with row stride ldp.
 constexpr int kFpTab = 4 * 4 * 2 * 16;
+template <int NCT>
 __global__ __launch_bounds__(256, 2) void k_fpca_xtb_bf16(
     const uint2 *__restrict__ selg, int64_t G64, int64_t M, int32_t nst,
     const uint32_t *__restrict__ lbf, const uint16_t *__restrict__ Bh,
     const uint16_t *__restrict__ Bl, int64_t ldt, int32_t ncg, int32_t S, int64_t mchunk,
     int32_t blkg, double *__restrict__ part, int32_t ldp) {
   __shared__ uint32_t tab[2][2][kFpTab + 1];  // [buf][hi/lo][entry], as the XTX kernel
-  __shared__ __attribute__((aligned(16))) uint16_t sB[2][2][64][64 + 8];  // [buf][hi/lo][col][snp]
+  constexpr int GW = 32 * NCT;  // columns per group
+  __shared__ __attribute__((aligned(16))) uint16_t sB[2][2][GW][64 + 8];  // [buf][hi/lo][col][snp]
   const int t = threadIdx.x, w = t >> 6, lane = t & 63, h = lane >> 5, r = lane & 31;
   const int64_t items = (int64_t)nst * S * ncg;
   const int64_t npad = (int64_t)nst * kTileS;
@@ -750,7 +752,7 @@ __global__ __launch_bounds__(256, 2) void k_fpca_xtb_bf16(
     const int64_t m_end = min(M, m_begin + mchunk);
     if (m_begin >= m_end) {  // an empty split (M not filling S chunks): zero partial
       double *out = part + ((int64_t)s * npad + (int64_t)st * kTileS) * ldp;
-      for (int e = t; e < kTileS * 64; e += 256) out[(int64_t)(e >> 6) * ldp + 64 * cg + (e & 63)] = 0.0;
+      for (int e = t; e < kTileS * GW; e += 256) out[(int64_t)(e / GW) * ldp + GW * cg + e % GW] = 0.0;
       continue;
     }
     const int32_t ng = (int32_t)((m_end - m_begin + kGrp - 1) / kGrp);
@@ -777,30 +779,32 @@ __global__ __launch_bounds__(256, 2) void k_fpca_xtb_bf16(
         tab[buf][1][e] = (e0 >> 16) | (e1 & 0xFFFF0000u);
       }
     };
-    // panel chunk of group gi: 2 x 64 columns x 64 SNPs = 1024 uint4, four per thread
-    // (u = 0, 1: hi rows t >> 3 and 32 + (t >> 3); u = 2, 3: the same lo rows)
+    // panel chunk of group gi: 2 x GW columns x 64 SNPs, 2 NCT uint4 per thread (hi and
+    // lo rows 32u + (t >> 3), u < NCT)
     const int pc = t >> 3, pch = t & 7;
-    const uint16_t *ph0 = Bh + (int64_t)(64 * cg + pc) * ldt + m_begin + pch * 8;
-    const uint16_t *pl0 = Bl + (int64_t)(64 * cg + pc) * ldt + m_begin + pch * 8;
+    const uint16_t *ph0 = Bh + (int64_t)(GW * cg + pc) * ldt + m_begin + pch * 8;
+    const uint16_t *pl0 = Bl + (int64_t)(GW * cg + pc) * ldt + m_begin + pch * 8;
     const int64_t p32 = 32 * ldt;
-    uint4 pv0, pv1, pv2, pv3;
+    fp_u32x4 pvh[NCT], pvl[NCT];
     auto pan_fetch = [&](int32_t gi) {
       const int64_t o = (int64_t)gi * kGrp;
-      pv0 = *(const uint4 *)(ph0 + o);
-      pv1 = *(const uint4 *)(ph0 + p32 + o);
-      pv2 = *(const uint4 *)(pl0 + o);
-      pv3 = *(const uint4 *)(pl0 + p32 + o);
+#pragma unroll
+      for (int u = 0; u < NCT; ++u) {
+        pvh[u] = *(const fp_u32x4 *)(ph0 + u * p32 + o);
+        pvl[u] = *(const fp_u32x4 *)(pl0 + u * p32 + o);
+      }
     };
     auto pan_store = [&](int buf) {
-      *(uint4 *)&sB[buf][0][pc][pch * 8] = pv0;
-      *(uint4 *)&sB[buf][0][32 + pc][pch * 8] = pv1;
-      *(uint4 *)&sB[buf][1][pc][pch * 8] = pv2;
-      *(uint4 *)&sB[buf][1][32 + pc][pch * 8] = pv3;
-    };
-    fp_f32x16 acc[2];
-    double dacc[2][16];
 #pragma unroll
-    for (int ct = 0; ct < 2; ++ct)
+      for (int u = 0; u < NCT; ++u) {
+        *(fp_u32x4 *)&sB[buf][0][32 * u + pc][pch * 8] = pvh[u];
+        *(fp_u32x4 *)&sB[buf][1][32 * u + pc][pch * 8] = pvl[u];
+      }
+    };
+    fp_f32x16 acc[NCT];
+    double dacc[NCT][16];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct)
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         acc[ct][q] = 0.f;
@@ -808,7 +812,7 @@ __global__ __launch_bounds__(256, 2) void k_fpca_xtb_bf16(
       }
     auto promote = [&]() {
 #pragma unroll
-      for (int ct = 0; ct < 2; ++ct)
+      for (int ct = 0; ct < NCT; ++ct)
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
           dacc[ct][q] += (double)acc[ct][q];
@@ -846,7 +850,7 @@ __global__ __launch_bounds__(256, 2) void k_fpca_xtb_bf16(
         }
         const fp_bf16x8 ah = __builtin_bit_cast(fp_bf16x8, xh), al = __builtin_bit_cast(fp_bf16x8, xl);
 #pragma unroll
-        for (int ct = 0; ct < 2; ++ct) {
+        for (int ct = 0; ct < NCT; ++ct) {
           const fp_bf16x8 bh = *(const fp_bf16x8 *)&sB[buf][0][32 * ct + r][16 * s4 + 8 * h];
           const fp_bf16x8 bl = *(const fp_bf16x8 *)&sB[buf][1][32 * ct + r][16 * s4 + 8 * h];
           acc[ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bl, acc[ct], 0, 0, 0);
@@ -867,11 +871,11 @@ __global__ __launch_bounds__(256, 2) void k_fpca_xtb_bf16(
     promote();
     double *out = part + ((int64_t)s * npad + (int64_t)st * kTileS) * ldp;
 #pragma unroll
-    for (int ct = 0; ct < 2; ++ct)
+    for (int ct = 0; ct < NCT; ++ct)
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int row = 32 * w + 8 * (q >> 2) + 4 * h + (q & 3);
-        out[(int64_t)row * ldp + 64 * cg + 32 * ct + r] = dacc[ct][q];
+        out[(int64_t)row * ldp + GW * cg + 32 * ct + r] = dacc[ct][q];
       }
   }
 }
@@ -1333,8 +1337,11 @@ static int launch_xa_bf16(eig_ctx *ctx, const Plan &P, const uint32_t *lbf, uint
 static int launch_xtb_bf16(eig_ctx *ctx, const Plan &P, const uint2 *selg, const uint32_t *lbf,
                            uint16_t *th, uint16_t *tl, int64_t Mpad, const double *B, int64_t ldb,
                            int32_t C, double *part, size_t part_doubles, double *out, int32_t ldo,
-                           double scale, int64_t slots, hipStream_t st) {
-  const int32_t ncg = (C + 63) / 64, cp = 64 * ncg;
+                           double scale, const int64_t slots[2], hipStream_t st) {
+  // 32-column groups for a panel of up to 32 columns (the power iteration's at L <= 32),
+  // else 64-column groups, which decode X^T once per 64 columns
+  const int nct = C <= 32 ? 1 : 2, gw = 32 * nct;
+  const int32_t ncg = (C + gw - 1) / gw, cp = gw * ncg;
   hipLaunchKernelGGL(k_fpca_split_t, dim3((unsigned)(Mpad / 32), (unsigned)(cp / 32)), dim3(256),
                      0, st, B, P.M, ldb, C, Mpad, cp, th, tl);
   const int64_t G64 = Mpad / kGrp;
@@ -1347,7 +1354,7 @@ static int launch_xtb_bf16(eig_ctx *ctx, const Plan &P, const uint2 *selg, const
     c = ((c + kGrp - 1) / kGrp) * kGrp;
     const int64_t ns = (P.M + c - 1) / c;
     if (ns != sp) continue;
-    const double rounds = (double)((P.nst * ns * ncg + slots - 1) / slots);
+    const double rounds = (double)((P.nst * ns * ncg + slots[nct - 1] - 1) / slots[nct - 1]);
     const double cost = rounds * (double)c / (double)P.M + 0.001 * (double)ns;
     if (best < 0.0 || cost < best - 1e-12) {
       best = cost;
@@ -1356,10 +1363,15 @@ static int launch_xtb_bf16(eig_ctx *ctx, const Plan &P, const uint2 *selg, const
     }
   }
   const int64_t items = (int64_t)P.nst * nsplit * ncg;
-  const unsigned grid = (unsigned)std::min<int64_t>(items, slots);
-  hipLaunchKernelGGL(k_fpca_xtb_bf16, dim3(grid), dim3(256), 0, st, selg, G64, P.M, P.nst, lbf,
-                     (const uint16_t *)th, (const uint16_t *)tl, Mpad, ncg, (int32_t)nsplit, chunk,
-                     8, part, cp);
+  const unsigned grid = (unsigned)std::min<int64_t>(items, slots[nct - 1]);
+  if (nct == 1)
+    hipLaunchKernelGGL(k_fpca_xtb_bf16<1>, dim3(grid), dim3(256), 0, st, selg, G64, P.M, P.nst,
+                       lbf, (const uint16_t *)th, (const uint16_t *)tl, Mpad, ncg, (int32_t)nsplit,
+                       chunk, 8, part, cp);
+  else
+    hipLaunchKernelGGL(k_fpca_xtb_bf16<2>, dim3(grid), dim3(256), 0, st, selg, G64, P.M, P.nst,
+                       lbf, (const uint16_t *)th, (const uint16_t *)tl, Mpad, ncg, (int32_t)nsplit,
+                       chunk, 8, part, cp);
   const int64_t tot = (int64_t)P.npad * cp;
   hipLaunchKernelGGL(k_fpca_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
                      (const double *)part, (int32_t)nsplit, (int64_t)P.npad, cp, scale, out, ldo);
@@ -1422,7 +1434,7 @@ extern "C" int eig_fpca_form(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint
   const size_t o_cb = add(sizeof(double) * kJacMax * kJacMax);
   const size_t o_ev = add(sizeof(double) * (size_t)K * npad);
   // split-BF16 form: levels, the X A panel (32 * ceil(L/32) columns) and the X^T B
-  // panel (64 * ceil(C/64) columns, C <= (I+1)L) as transposed bf16 hi / lo
+  // panel (up to 64 * ceil(C/64) columns, C <= (I+1)L) as transposed bf16 hi / lo
   const int64_t Mpad = ((M + kGrp - 1) / kGrp) * kGrp;
   const int32_t cxa = 32 * ((L + 31) / 32), cxb = 64 * ((Ctot + 63) / 64);
   const size_t o_lbf = bf ? add(sizeof(uint32_t) * 4 * M) : 0;
@@ -1460,7 +1472,7 @@ extern "C" int eig_fpca_form(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint
   }
   EIG_HIP(ctx, hipGetLastError());
   Plan P{M, nst, npad, (const uint32_t *)ctx->sel.p, lf};
-  int64_t bslots = 0;
+  int64_t bslots[2] = {0, 0};  // resident X^T B blocks, 32- and 64-column groups
   int bxa_grid = 0;
   if (bf) {
     hipLaunchKernelGGL(k_fpca_levbf, dim3((unsigned)((4 * M + 255) / 256)), dim3(256), 0, st,
@@ -1468,8 +1480,11 @@ extern "C" int eig_fpca_form(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint
     EIG_HIP(ctx, hipGetLastError());
     EIG_TRY(eig_ensure_selg(ctx));
     int occ = 0;
-    EIG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fpca_xtb_bf16, 256, 0));
-    bslots = (int64_t)std::max(occ, 1) * std::max(ctx->num_cu, 1);
+    EIG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fpca_xtb_bf16<1>, 256, 0));
+    bslots[0] = (int64_t)std::max(occ, 1) * std::max(ctx->num_cu, 1);
+    occ = 0;
+    EIG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fpca_xtb_bf16<2>, 256, 0));
+    bslots[1] = (int64_t)std::max(occ, 1) * std::max(ctx->num_cu, 1);
     occ = 0;
     EIG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fpca_xa_bf16<2>, 256, 0));
     bxa_grid = (int)std::min<int64_t>((M + 127) / 128, (int64_t)std::max(occ, 1) * ctx->num_cu);
@@ -1498,7 +1513,7 @@ extern "C" int eig_fpca_form(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint
       (int64_t)(uintptr_t)ctx->sel.p, (int64_t)(uintptr_t)ctx->selg.p, (int64_t)(uintptr_t)base,
       (int64_t)off, (int64_t)o_lf, (int64_t)o_a0, (int64_t)o_a1, (int64_t)o_q, (int64_t)o_part,
       (int64_t)o_lbf, (int64_t)o_ta, (int64_t)o_tb, M, nst, K, L, I, (int64_t)bf, inv_bits,
-      bf ? (int64_t)bslots * 100000 + bxa_grid
+      bf ? ((int64_t)bslots[0] * 100000 + bslots[1]) * 100000 + bxa_grid
          : (int64_t)F.xa[CTL].grid * 1000 + F.xtb[CTL].S};
   (void)old;
   const bool same = F.gexec && F.gsig == sig;
