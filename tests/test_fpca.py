@@ -96,7 +96,10 @@ def test_fpca_builtin_generator_reaches_exact_eigenpairs(eng, N, M, K, L, I):
     assert np.array_equal(ev, ev2) and np.array_equal(vec, vec2)
 
 
-@pytest.mark.parametrize("N,M,K,L,I", [(300, 20000, 10, 20, 10), (200, 4000, 3, 8, 5)])
+# L=40: 64-column panel products in the power iteration; L=10, I=2: a B product of
+# at most 32 columns (its 32-column X^T B group)
+@pytest.mark.parametrize("N,M,K,L,I", [(300, 20000, 10, 20, 10), (200, 4000, 3, 8, 5),
+                                       (200, 4000, 4, 40, 2), (150, 3000, 3, 10, 2)])
 def test_fpca_split_bf16_panels_match_fp64(eng, N, M, K, L, I):
     """The split-BF16 panel products (4 products, FP32 within 512-sample/SNP blocks)
     against the FP64 form with the same start panel.  Bars: eigenvalues 1e-6
