@@ -1610,8 +1610,18 @@ extern "C" int eig_fpca_form(eig_ctx *ctx, int32_t K, int32_t L, int32_t I, uint
   if (r < K) return eig_set_err(ctx, EIG_E_NUMERIC, "fpca: Krylov basis rank %d < K", r);
   // ---- B = X^T U  (npad x r, ld ldq), then the top-K left singular vectors of B
   if (bf) {
-    EIG_TRY(launch_xtb_bf16(ctx, P, (const uint2 *)ctx->selg.p, lbf, tbh, tbl, Mpad, src, ldq, r,
-                            part, part_doubles, Bm, (int32_t)ldq, 1.0, bslots, st));
+    // 64-column groups, and a remainder of at most 32 columns in a 32-column group, which
+    // pads less (a 64-column group took 20 ms, a 32-column one 12 ms at N=100,000 x
+    // M=500,000) when its 32 output columns stay within the row stride ldq
+    const int32_t rem = r % 64;
+    const int32_t r64 = (rem != 0 && rem <= 32 && r - rem + 32 <= ldq) ? r - rem : r;
+    if (r64 > 0)
+      EIG_TRY(launch_xtb_bf16(ctx, P, (const uint2 *)ctx->selg.p, lbf, tbh, tbl, Mpad, src, ldq,
+                              r64, part, part_doubles, Bm, (int32_t)ldq, 1.0, bslots, st));
+    if (r64 < r)
+      EIG_TRY(launch_xtb_bf16(ctx, P, (const uint2 *)ctx->selg.p, lbf, tbh, tbl, Mpad, src + r64,
+                              ldq, r - r64, part, part_doubles, Bm + r64, (int32_t)ldq, 1.0, bslots,
+                              st));
   } else {
     // in 32-column pieces: the wide instances hold all r columns' accumulators and run
     // one wave per SIMD, at about half the FP64 rate of k_fpca_xtb<2> (four waves per
