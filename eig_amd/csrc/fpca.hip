@@ -617,7 +617,10 @@ __global__ __launch_bounds__(256, 2) void k_fpca_xa_bf16_pf(
     const uint16_t *__restrict__ Ah, const uint16_t *__restrict__ Al, int64_t ldt,
     double *__restrict__ Y, int64_t ldy, int32_t C, int32_t blkt) {
   constexpr int kPE = 2 * 32 * CT * 16 / 256;  // panel uint4 per thread per tile
-  __shared__ uint32_t tt[2][128 * 17 + 1];
+  // tables entry-major (entry c of SNP s at c * 128 + s): the 32 lanes of a half-wave
+  // are 32 SNPs, so their lookups fall in 32 different banks whatever their codes; the
+  // two tables 8196 bytes apart, which no paired LDS load spans
+  __shared__ uint32_t tt[2][16 * 128 + 1];
   __shared__ __attribute__((aligned(16))) uint16_t sA[2][2][32 * CT][128 + 8];  // [buf][hi/lo]
   const int t = threadIdx.x, w = t >> 6, lane = t & 63, h = lane >> 5, r = lane & 31;
   const int64_t ntiles = (M + 127) / 128;
@@ -627,18 +630,18 @@ __global__ __launch_bounds__(256, 2) void k_fpca_xa_bf16_pf(
     const int64_t mb = tile * 128;
     __syncthreads();  // the previous tile's table reads are done
     for (int e = t; e < 128 * 16; e += 256) {
-      const int sl = e >> 4, c = e & 15;
+      const int sl = e & 127, c = e >> 7;
       const int64_t m = mb + sl;
       uint32_t e0 = 0u, e1 = 0u;
       if (m < M) {
         e0 = lbf[4 * m + (c & 3)];
         e1 = lbf[4 * m + (c >> 2)];
       }
-      tt[0][sl * 17 + c] = (e0 & 0xFFFFu) | (e1 << 16);
-      tt[1][sl * 17 + c] = (e0 >> 16) | (e1 & 0xFFFF0000u);
+      tt[0][e] = (e0 & 0xFFFFu) | (e1 << 16);
+      tt[1][e] = (e0 >> 16) | (e1 & 0xFFFF0000u);
     }
     const int64_t m = mb + 32 * w + r;  // this lane's SNP
-    const uint32_t rowoff = (uint32_t)((32 * w + r) * 17);
+    const uint32_t rowoff = (uint32_t)(32 * w + r);
     auto fetch = [&](int32_t st) {
 #pragma unroll
       for (int j = 0; j < kPE; ++j) {
@@ -692,8 +695,8 @@ __global__ __launch_bounds__(256, 2) void k_fpca_xa_bf16_pf(
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const uint32_t c = (bits >> (4 * q)) & 15u;
-          xh[q] = tt[0][rowoff + c];
-          xl[q] = tt[1][rowoff + c];
+          xh[q] = tt[0][c * 128 + rowoff];
+          xl[q] = tt[1][c * 128 + rowoff];
         }
         const fp_bf16x8 ah = __builtin_bit_cast(fp_bf16x8, xh), al = __builtin_bit_cast(fp_bf16x8, xl);
 #pragma unroll
