@@ -13,10 +13,13 @@
 // eigenvectors, S^2 / ncols the eigenvalues (kjg_fpca.c:79-95).
 //
 // The two panel products read X straight from the 2-bit sample-tile layout and
-// multiply on FP64 MFMA (16x16x4); both are persistent kernels whose grid and
+// multiply on FP64 MFMA (16x16x4), or, in the split-BF16 form, on BF16 MFMA
+// (32x32x16) over hi/lo splits; both are persistent kernels whose grid and
 // split-K factor are picked by a timing tuner on the first call for a shape, and
 // the 2I+1 products of the power iteration are replayed as one captured graph.
-// The tall-skinny Gram and basis products are plain GEMMs (rocBLAS).
+// The instances of up to 32 columns load each step's operands while the previous
+// step's MFMAs run (two LDS buffers, one barrier per step).  The tall-skinny Gram
+// and basis products run on the library's FP64 MFMA GEMM (gemm.hip).
 
 #include <algorithm>
 #include <cmath>
