@@ -1002,7 +1002,7 @@ struct FpcaState {
   // scale; a call whose signature differs re-captures
   std::vector<int64_t> gsig;
   FpcaTune xa[kMaxCT + 1], xtb[kMaxCT + 1];
-  // held for the whole call: the graph, the tuners and the rocBLAS handle are shared by
+  // held for the whole call: the graph and the tuners are shared by
   // every context on the device
   std::mutex mu;
 };
